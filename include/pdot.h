@@ -1,0 +1,169 @@
+/*
+ * pdot.h -- C ABI of libpdot_hip.so, the MI355X (gfx950) hot path of PDOT:
+ * restarted adaptive PDHG for discrete optimal transport
+ *     min <C,X>  s.t.  X 1 = a,  X^T 1 = b,  X >= 0.
+ *
+ * The library owns all device memory and one HIP stream per context.  Host
+ * pointers are borrowed for the duration of a call.  Matrices cross the
+ * boundary column-major with leading dimension n (element (i,j) at i + j*n),
+ * as Julia's Matrix{Float64} lays them out.  Every call returns 0 on success
+ * or a negative PDOT_E_* code; pdot_last_error() then describes the failure.
+ * A context is not thread-safe; distinct contexts may be used concurrently.
+ *
+ * The host control logic (restart scheme, termination, primal-weight update,
+ * logs) stays with the caller; the library performs the n*m work and the
+ * per-iteration step-size recurrence on the device.  Each entry point names
+ * the reference code it replaces (paths relative to the PDOT.jl tree).
+ */
+#ifndef PDOT_H
+#define PDOT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pdot_ctx pdot_ctx;
+
+enum {
+    PDOT_OK = 0,
+    PDOT_E_ARG = -1,   /* invalid argument (sizes, pointers, slot state) */
+    PDOT_E_HIP = -2,   /* a HIP runtime call failed */
+    PDOT_E_STATE = -3, /* call not valid in the current solver state */
+    PDOT_E_NOMEM = -4  /* device allocation failed */
+};
+
+enum { PDOT_STORAGE_F64 = 0, PDOT_STORAGE_F32 = 1 };
+enum { PDOT_POLICY_ADAPTIVE = 0, PDOT_POLICY_CONSTANT = 1 };
+enum { PDOT_POINT_CURRENT = 0, PDOT_POINT_AVERAGE = 1 };
+
+/* Device-resident solver scalars: the scalar fields of CuPdhgSolverState
+ * (src/solver_gpu.jl:23-36) and of CuSolutionWeightedAverage
+ * (src/saddle_point_gpu.jl:52-65), plus the outputs of the last pass.
+ * cumulative_kkt_passes counts step passes only; the caller adds the 2.0 per
+ * evaluation (src/solver_gpu.jl:484, 534). */
+typedef struct pdot_solver_state {
+    double step_size;             /* solver_state.step_size (eta) */
+    double primal_weight;         /* solver_state.primal_weight (omega) */
+    double cumulative_kkt_passes; /* +1 per PDHG pass (solver_gpu.jl:336) */
+    double sum_weights;           /* sum_*_solution_weights (all three are equal) */
+    int64_t total_number_iterations;
+    int64_t avg_count;            /* *_solutions_count (all three are equal) */
+    int64_t take_step_calls;      /* take_step! invocations so far */
+    int64_t error_call;           /* take_step! call that set numerical_error, else 0 */
+    double last_interaction;      /* compute_interaction_and_movement (solver_gpu.jl:267-288) */
+    double last_movement;
+    double last_step_limit;
+    double last_dx2;              /* ||Delta X||_F^2 of the last pass */
+    double last_dot;              /* dr'du + dc'dv of the last pass */
+    double last_dy2;              /* ||[du; dv]||^2 of the last pass */
+    double reduction_exponent;    /* AdaptiveStepsizeParams (solver_gpu.jl:2-5) */
+    double growth_exponent;
+    int32_t step_policy;          /* PDOT_POLICY_* */
+    int32_t numerical_error;      /* solver_state.numerical_error; halts further passes */
+    int32_t gzero;                /* the primal gradient is still the initial zero matrix */
+    int32_t cur_slot;             /* internal: slot holding the current point */
+    int32_t avg_slot;             /* internal: slot holding the average (valid if avg_count > 0) */
+    int32_t pending;              /* internal: a speculative pass awaits commit */
+    int32_t spare0, spare1;       /* internal: slots written by the pending pass */
+} pdot_solver_state;
+
+/* Reductions of one point (X, u, v, r, c, G) with G = (C - u 1^T) - 1 v^T,
+ * from which the caller forms ConvergenceInformation (src/utils_gpu.jl:68-129)
+ * and the weighted KKT residual (src/saddle_point_gpu.jl:299-330). */
+typedef struct pdot_point_stats {
+    double primal_objective;  /* <C, X>                       utils_gpu.jl:84 */
+    double x_inf;             /* ||X||_inf                    utils_gpu.jl:102 */
+    double x_sq;              /* ||X||_F^2                    utils_gpu.jl:103 */
+    double rc_inf;            /* ||min(G,0)||_inf             utils_gpu.jl:52 */
+    double rc_sq;             /* ||min(G,0)||_F^2             utils_gpu.jl:108 */
+    double p_inf;             /* ||[a-r; b-c]||_inf           utils_gpu.jl:94 */
+    double p_sq;              /* ||[a-r; b-c]||_2^2           utils_gpu.jl:95 */
+    double dual_objective;    /* a'u + b'v                    utils_gpu.jl:54 */
+    double u_inf;             /* ||u||_inf  (= ||[u;u]||_inf, utils_gpu.jl:115) */
+    double u_sq;              /* ||u||_2^2                    utils_gpu.jl:116 */
+    double y_sq;              /* ||[u; v]||_2^2               saddle_point_gpu.jl:765 */
+} pdot_point_stats;
+
+/* Create a context for an n x m problem on HIP device `device`, storing C, X
+ * and the running average X_avg in FP64 (PDOT_STORAGE_F64) or FP32
+ * (PDOT_STORAGE_F32, arithmetic and duals stay FP64).  Replaces the
+ * CUDA.zeros allocations of src/solver_gpu.jl:428-471. */
+int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t device);
+void pdot_destroy(pdot_ctx* ctx);
+const char* pdot_last_error(const pdot_ctx* ctx);
+
+/* Host-to-device copy of the problem, ot_cpu_to_gpu (src/io.jl:18-36).
+ * C: n*m column-major FP64; a: n; b: m. */
+int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b);
+
+/* Reset the iterate: X = 0, r = c = 0, empty average, counters 0
+ * (src/solver_gpu.jl:428-441, 476-496).  u0/v0 NULL gives the reference's zero
+ * start with G = 0; otherwise a warm start (u0, v0) with G0 = (C - u0) - v0.
+ * step_policy/exponents select AdaptiveStepsizeParams or ConstantStepsizeParams;
+ * step_size and primal_weight are the initial eta and omega. */
+int pdot_set_start(pdot_ctx* ctx, const double* u0, const double* v0, int32_t step_policy,
+                   double reduction_exponent, double growth_exponent, double step_size,
+                   double primal_weight);
+
+/* Run `count` take_step! calls (src/solver_gpu.jl:290-367 adaptive, :370-410
+ * constant) with the step-size recurrence on the device and no host round
+ * trip; a pass pending from pdot_eval is committed first.  Returns once the
+ * state is committed; `out` (may be NULL) receives it.  A numerical error halts
+ * the remaining passes and is reported in out->numerical_error/error_call. */
+int pdot_run_steps(pdot_ctx* ctx, int64_t count, pdot_solver_state* out);
+
+/* Evaluate the committed point and its running average in one streaming pass
+ * (compute_convergence_information, src/utils_gpu.jl:68-129, and the
+ * n*m parts of compute_weight_kkt_residual, src/saddle_point_gpu.jl:299-330),
+ * while speculatively performing the next take_step! into spare buffers.
+ * The pass is committed by the next pdot_run_steps/pdot_eval and discarded by
+ * pdot_restart or pdot_discard.  `average` is zeroed when the average is empty.
+ * `state` (may be NULL) receives the pre-pass state. */
+int pdot_eval(pdot_ctx* ctx, pdot_point_stats* current, pdot_point_stats* average,
+              pdot_solver_state* state);
+
+/* Discard a pending speculative pass (no-op if none). */
+int pdot_discard(pdot_ctx* ctx);
+
+/* Restart, src/saddle_point_gpu.jl:639-696: discard any pending pass; if
+ * to_average, the average becomes the current point (X, u, v, r, c and,
+ * implicitly, G); the average is reset (:667); omega is set to
+ * new_primal_weight (compute_new_primal_weight, :701-728, runs on the host). */
+int pdot_restart(pdot_ctx* ctx, int32_t to_average, double new_primal_weight);
+
+/* Read or overwrite the device solver scalars. */
+int pdot_get_state(pdot_ctx* ctx, pdot_solver_state* out);
+int pdot_set_state(pdot_ctx* ctx, const pdot_solver_state* in);
+
+/* Device-to-host copy of the current (PDOT_POINT_CURRENT) or average point,
+ * gpu_to_cpu! (src/io.jl:41-53).  X: n*m column-major FP64 (may be NULL),
+ * u: n, v: m (may be NULL).  FP32 storage is widened on the device. */
+int pdot_download(pdot_ctx* ctx, int32_t which, double* X, double* u, double* v);
+
+/* Timing hook: run `count` take_step! calls like pdot_run_steps while
+ * recording HIP events on the context's stream around every launch of the
+ * streaming step kernel.  step_kernel_ms = summed step-kernel time,
+ * total_ms = elapsed time of the whole sequence. */
+int pdot_time_steps(pdot_ctx* ctx, int64_t count, double* step_kernel_ms, double* total_ms);
+
+/* Use a captured hipGraph for pdot_run_steps sequences (1) or plain stream
+ * launches (0, the default). */
+int pdot_use_graphs(pdot_ctx* ctx, int32_t enable);
+
+/* Launch geometry of the streaming step kernel: rows per block, column
+ * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the
+ * default choice (used by the launch-shape tuner); 0 keeps the default. */
+int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint);
+int pdot_get_launch(pdot_ctx* ctx, int32_t* rows_per_block, int32_t* col_blocks, int32_t* cols_per_block,
+                    int32_t* blocks);
+
+/* Bytes of device memory held by the context. */
+int64_t pdot_device_bytes(const pdot_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PDOT_H */

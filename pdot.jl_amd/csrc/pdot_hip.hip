@@ -1,0 +1,1274 @@
+// pdot_hip.hip -- gfx950 (MI355X, CDNA4) kernels and the C ABI of
+// libpdot_hip.so: the PDHG hot path of PDOT.jl re-designed as HBM-streaming
+// kernels.  See include/pdot.h for the interface contract.
+//
+// Data layout in HBM (one context = one n x m problem):
+//   C, and four "slots" each holding a point (X, u, v, r, c).  Matrices are
+//   column-major with a padded leading dimension ld = roundup(n, 64), FP64 or
+//   FP32.  The current point and the running average each live in a slot; a
+//   PDHG pass reads the current slot (and the average slot) and writes the new
+//   point and the new average into two spare slots, so a pass can be
+//   committed, or discarded (restart, termination, numerical error), by
+//   changing slot indices only.  G = (C - u 1^T) - 1 v^T is never stored: it
+//   is recomputed in registers from C, u and v, exactly as
+//   src/solver_gpu.jl:249-250 forms it.
+//
+// Per PDHG pass (one take_step!, src/solver_gpu.jl:290-367):
+//   step_kernel    streams C, X, X_avg -> X+, X_avg+ and emits per-block
+//                  partial row sums, column sums and ||Delta||^2 of
+//                  Delta = max(X - tau G, 0) - X (5 n*m passes in total).
+//   reduce_kernel  finishes the row/column sums (fixed order), applies the
+//                  dual update, the averaging of u, v, r, c and the vector
+//                  parts of interaction and movement.
+//   the controller (the adaptive step-size recurrence, the replay of a
+//                  rejected step and the commit) runs in the last block of
+//                  reduce_kernel to finish (an atomic completion counter), or
+//                  in commit_kernel for a pass left pending by an evaluation.
+// Evaluation of the point and its average (src/utils_gpu.jl:68-129) is fused
+// into the step kernel as extra reductions of the values it already loads.
+//
+// All reductions have a fixed order, so results are bitwise reproducible.
+// Compiled with -ffp-contract=off: every expression rounds as the reference's
+// broadcasts do.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "pdot.h"
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kWaves = 4;
+constexpr int kBlock = kWave * kWaves;
+constexpr int kRedBlock = kWave;  // reduce kernel: one wave per block
+constexpr int kChunk = 64;        // columns per LDS flush of column sums
+constexpr int kLdAlign = 64;      // leading-dimension alignment (elements)
+constexpr int kMatStats = 10;     // per-block matrix eval reductions
+constexpr int kVecStats = 12;     // per-block vector eval reductions
+constexpr int kEvalOut = kMatStats + kVecStats;
+#ifndef PDOT_STEP_UNROLL
+#define PDOT_STEP_UNROLL 1
+#endif
+constexpr int kUnroll = PDOT_STEP_UNROLL;  // columns whose loads are issued together
+
+struct SlotPtrs {
+    void* X[4];
+    double* u[4];
+    double* v[4];
+    double* r[4];
+    double* c[4];
+};
+
+struct StepParams {
+    const void* C;
+    SlotPtrs s;
+    const pdot_solver_state* st;
+    double* rowpart;
+    double* colpart;
+    double* sqpart;
+    double* evalpart;
+    int64_t n, m, ld;
+    int32_t TC, Cb, R;
+};
+
+struct ReduceParams {
+    SlotPtrs s;
+    const double* a;
+    const double* b;
+    pdot_solver_state* st;
+    unsigned int* counter;
+    const double* rowpart;
+    const double* colpart;
+    const double* sqpart;
+    double* redpart;
+    double* evalvec;
+    int64_t n, m, ld;
+    int32_t Cb, R, NBr, NBc, nred, nstep, sqchunk;
+};
+
+// ------------------------------------------------------------------ helpers
+
+// Julia's max(x, 0.0) / min(x, 0.0) / min(x, y) propagate NaN.
+__device__ __forceinline__ double jl_max0(double x) { return x != x ? x : (x > 0.0 ? x : 0.0); }
+__device__ __forceinline__ double jl_min0(double x) { return x != x ? x : (x < 0.0 ? x : 0.0); }
+__device__ __forceinline__ double jl_min(double x, double y) {
+    if (x != x || y != y) return __longlong_as_double(0x7ff8000000000000LL);
+    return x < y ? x : y;
+}
+// max of non-negative values that keeps a NaN once seen
+__device__ __forceinline__ double nanmax(double acc, double t) { return (acc != acc || t != t || t > acc) ? (acc != acc ? acc : t) : acc; }
+
+// (k+1)^e as Julia's Int^Float64: integral exponents by repeated products.
+__device__ double pow_int_base(double x, double e) {
+    if (e == floor(e) && fabs(e) <= 64.0) {
+        int k = (int)fabs(e);
+        double r = 1.0;
+        for (int t = 0; t < k; ++t) r *= x;
+        return e < 0 ? 1.0 / r : r;
+    }
+    return pow(x, e);
+}
+
+// DPP lane movement of a double (two 32-bit halves); lanes outside row_mask
+// or reading an invalid source receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+    int lo = __double2loint(x);
+    int hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed DPP tree; every lane gets the
+// total.  Must be called with all lanes active.
+__device__ __forceinline__ double wave_sum(double x) {
+    x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+    x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+    x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+    x += dpp_f64<0x140, 0xF>(x);  // row_mirror
+    x += dpp_f64<0x142, 0xA>(x);  // row_bcast:15
+    x += dpp_f64<0x143, 0xC>(x);  // row_bcast:31
+    return readlane_f64(x, 63);
+}
+__device__ __forceinline__ double wave_max(double x) {
+    x = nanmax(x, dpp_f64<0xB1, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x4E, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x141, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x140, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x142, 0xA>(x));
+    x = nanmax(x, dpp_f64<0x143, 0xC>(x));
+    return readlane_f64(x, 63);
+}
+
+// Deterministic block-wide sum/max (256 threads); result valid in all threads.
+__device__ double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = red[tid] + red[tid + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+__device__ double block_max(double v, double* red) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = v;
+    __syncthreads();
+    for (int s = kBlock / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = nanmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// --------------------------------------------------------------- controller
+
+// The two slots a pass writes: the first slots holding neither the current
+// point nor the (non-empty) average.
+__device__ __forceinline__ void choose_spares(pdot_solver_state& st) {
+    int used0 = st.cur_slot, used1 = st.avg_count > 0 ? st.avg_slot : st.cur_slot;
+    int k = 0;
+    for (int s = 0; s < 4; ++s) {
+        if (s == used0 || s == used1) continue;
+        if (k == 0) st.spare0 = s;
+        else if (k == 1) st.spare1 = s;
+        ++k;
+    }
+}
+
+// Commit the pending pass: take_step!'s acceptance test and step-size
+// recurrence (src/solver_gpu.jl:296-366), its replay of a rejected pass (the
+// replay recomputes an identical pass, so only the scalars change), and the
+// swap of the spare slots into the current point and the average
+// (update_solution_in_solver_state!, :237-264, whose averaging weight is the
+// pre-step eta).  A pass with interaction > 0 and movement == 0 sets
+// numerical_error and is not applied (:338-344); a NaN step-size limit, on
+// which the reference loops forever, is reported the same way.
+__device__ void controller_commit(pdot_solver_state& st, double dot, double yy, double dx2) {
+    st.pending = 0;
+    const double eta = st.step_size;
+    const double omega = st.primal_weight;
+    bool commit;
+    st.take_step_calls += 1;
+    st.last_dx2 = dx2;
+    st.last_dot = dot;
+    st.last_dy2 = yy;
+    if (st.step_policy == PDOT_POLICY_CONSTANT) {
+        st.cumulative_kkt_passes += 1;
+        commit = true;
+    } else {
+        const double interaction = fabs(dot);
+        const double nd = sqrt(dx2);
+        const double ny = sqrt(yy);
+        const double movement = 0.5 * omega * (nd * nd) + (0.5 / omega) * (ny * ny);
+        st.last_interaction = interaction;
+        st.last_movement = movement;
+        double L = __longlong_as_double(0x7ff0000000000000LL);
+        double eta_loc = eta;
+        bool done = false, replay = false, ne = false;
+        for (int pass = 0; pass < 4 && !done; ++pass) {
+            st.total_number_iterations += 1;
+            st.cumulative_kkt_passes += 1;
+            if (!replay) {
+                if (interaction > 0) {
+                    L = movement / interaction;
+                    if (movement == 0.0) {
+                        ne = true;
+                        break;
+                    }
+                } else {
+                    L = __longlong_as_double(0x7ff0000000000000LL);
+                }
+            }
+            if (eta_loc <= L) done = true;
+            const double k1 = (double)(st.total_number_iterations + 1);
+            const double first = (1 - 1 / pow_int_base(k1, st.reduction_exponent)) * L;
+            const double second = (1 + 1 / pow_int_base(k1, st.growth_exponent)) * eta_loc;
+            eta_loc = jl_min(first, second);
+            replay = true;
+            if (!done && eta_loc != eta_loc) {
+                ne = true;
+                break;
+            }
+        }
+        if (!done) ne = true;
+        st.last_step_limit = L;
+        st.step_size = eta_loc;
+        if (ne) {
+            st.numerical_error = 1;
+            st.error_call = st.take_step_calls;
+        }
+        commit = done;
+    }
+    if (commit) {
+        st.avg_slot = (st.avg_count == 0) ? st.spare0 : st.spare1;
+        st.cur_slot = st.spare0;
+        st.sum_weights += eta;
+        st.avg_count += 1;
+        st.gzero = 0;
+        choose_spares(st);
+    }
+}
+
+// Reduce the per-block scalars of the reduce kernel (dot, ||dy||^2, ||dX||^2
+// chunk) in a fixed order and commit.  Called by all 64 lanes of one wave.
+__device__ void reduce_and_commit(pdot_solver_state* sst, const double* redpart, int nred) {
+    const int tid = threadIdx.x;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int k = tid; k < nred; k += kWave) {
+        d0 += redpart[3 * k + 0];
+        d1 += redpart[3 * k + 1];
+        d2 += redpart[3 * k + 2];
+    }
+    d0 = wave_sum(d0);
+    d1 = wave_sum(d1);
+    d2 = wave_sum(d2);
+    if (tid == 0) controller_commit(*sst, d0, d1, d2);
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------- vectors
+
+template <typename T> struct VecT;
+template <> struct VecT<double> {
+    static constexpr int N = 2;
+    using type = double2;
+};
+template <> struct VecT<float> {
+    static constexpr int N = 4;
+    using type = float4;
+};
+
+template <typename V, typename T>
+__device__ __forceinline__ void unpack(const V& v, double* out);
+template <>
+__device__ __forceinline__ void unpack<double2, double>(const double2& v, double* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+}
+template <>
+__device__ __forceinline__ void unpack<float4, float>(const float4& v, double* o) {
+    o[0] = (double)v.x;
+    o[1] = (double)v.y;
+    o[2] = (double)v.z;
+    o[3] = (double)v.w;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_vec(T* p, const double* o);
+template <>
+__device__ __forceinline__ void store_vec<double>(double* p, const double* o) {
+    *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
+}
+template <>
+__device__ __forceinline__ void store_vec<float>(float* p, const double* o) {
+    *reinterpret_cast<float4*>(p) = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+}
+
+// ------------------------------------------------------------- step kernel
+
+// One PDHG pass over a tile of kWaves*64*VEC rows x TC columns; each lane owns
+// VEC consecutive rows and walks the tile's columns.  Row sums accumulate in
+// registers, column sums are reduced per column across the wave (DPP) and
+// across the block's waves (LDS).  With EVAL the same loads also produce the
+// evaluation reductions of the input point and of its average.
+template <typename T, bool EVAL, int UNROLL>
+__global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
+    constexpr int VEC = VecT<T>::N;
+    using V = typename VecT<T>::type;
+    constexpr int WROWS = kWave * VEC;
+    constexpr int TR = kWaves * WROWS;
+
+    __shared__ pdot_solver_state sst;
+    __shared__ double red[kBlock];
+    __shared__ double colbuf[kWaves][kChunk];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+
+    if (tid == 0) sst = *p.st;
+    __syncthreads();
+
+    const bool do_step = !sst.numerical_error;
+    if (!do_step && !EVAL) return;
+
+    const int rb = blockIdx.x % p.R;
+    const int cb = blockIdx.x / p.R;
+    const int64_t n = p.n, m = p.m, ld = p.ld;
+    const int64_t i0 = (int64_t)rb * TR + (int64_t)w * WROWS + (int64_t)lane * VEC;
+    const bool lane_active = i0 < n;
+    const int64_t j0 = (int64_t)cb * p.TC;
+    const int64_t j1 = (j0 + p.TC < m) ? j0 + p.TC : m;
+
+    const bool avg_empty = sst.avg_count == 0;
+    const int cs = sst.cur_slot;
+    const int as = avg_empty ? cs : sst.avg_slot;
+    const T* __restrict__ C = static_cast<const T*>(p.C);
+    const T* __restrict__ Xc = static_cast<const T*>(p.s.X[cs]);
+    const T* __restrict__ Xa = static_cast<const T*>(p.s.X[as]);
+    T* __restrict__ Xn = static_cast<T*>(p.s.X[sst.spare0]);
+    T* __restrict__ Xan = static_cast<T*>(p.s.X[sst.spare1]);
+    const double* __restrict__ vc = p.s.v[cs];
+    const double* __restrict__ va = p.s.v[as];
+
+    const double eta = sst.step_size;
+    const double omega = sst.primal_weight;
+    const double S = sst.sum_weights;
+    double nw = 1.0, aw = 1.0;
+    if (S > 0.0) {
+        aw = S / (S + eta);
+        nw = eta / S;
+    }
+    const bool gzero = sst.gzero != 0;
+
+    double ui[VEC], uai[VEC];
+    bool valid[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        valid[e] = (i0 + e) < n;
+        ui[e] = (lane_active && valid[e]) ? p.s.u[cs][i0 + e] : 0.0;
+        uai[e] = (EVAL && lane_active && valid[e]) ? p.s.u[as][i0 + e] : 0.0;
+    }
+
+    double rs[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) rs[e] = 0.0;
+    double dd = 0.0;
+    double ev[kMatStats];
+#pragma unroll
+    for (int k = 0; k < kMatStats; ++k) ev[k] = 0.0;
+
+    for (int64_t jc = j0; jc < j1; jc += kChunk) {
+        const int64_t jend = (jc + kChunk < j1) ? jc + kChunk : j1;
+        for (int64_t jb = jc; jb < jend; jb += UNROLL) {
+            V xv[UNROLL], cv[UNROLL], av[UNROLL];
+            double vj[UNROLL], vaj[UNROLL];
+#pragma unroll
+            for (int q = 0; q < UNROLL; ++q) {
+                const int64_t j = jb + q;
+                const bool colok = j < jend;
+                vj[q] = colok ? vc[j] : 0.0;
+                vaj[q] = (EVAL && colok) ? va[j] : 0.0;
+                if (colok && lane_active) {
+                    const int64_t idx = j * ld + i0;
+                    xv[q] = *reinterpret_cast<const V*>(Xc + idx);
+                    cv[q] = *reinterpret_cast<const V*>(C + idx);
+                    if (!avg_empty) av[q] = *reinterpret_cast<const V*>(Xa + idx);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < UNROLL; ++q) {
+                const int64_t j = jb + q;
+                const bool colok = j < jend;
+                double csl = 0.0;
+                if (colok && lane_active) {
+                    const int64_t idx = j * ld + i0;
+                    double x[VEC], c[VEC], xa[VEC], xn[VEC], xan[VEC];
+                    unpack<V, T>(xv[q], x);
+                    unpack<V, T>(cv[q], c);
+                    if (!avg_empty) unpack<V, T>(av[q], xa);
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        // G_ij = (C_ij - u_i) - v_j (solver_gpu.jl:249-250); 0 at start (:434)
+                        double g = (gzero || !valid[e]) ? 0.0 : (c[e] - ui[e]) - vj[q];
+                        if (EVAL) {
+                            ev[0] += c[e] * x[e];
+                            ev[1] = nanmax(ev[1], fabs(x[e]));
+                            ev[2] += x[e] * x[e];
+                            const double rc = jl_min0(g);
+                            ev[3] = nanmax(ev[3], fabs(rc));
+                            ev[4] += rc * rc;
+                            if (!avg_empty) {
+                                const double ga = valid[e] ? (c[e] - uai[e]) - vaj[q] : 0.0;
+                                ev[5] += c[e] * xa[e];
+                                ev[6] = nanmax(ev[6], fabs(xa[e]));
+                                ev[7] += xa[e] * xa[e];
+                                const double rca = jl_min0(ga);
+                                ev[8] = nanmax(ev[8], fabs(rca));
+                                ev[9] += rca * rca;
+                            }
+                        }
+                        // Delta = max(X - (eta/omega) G, 0) - X  (solver_gpu.jl:139-141)
+                        double d = x[e] - (eta / omega) * g;
+                        d = jl_max0(d);
+                        double xnew;
+                        if constexpr (sizeof(T) == 8) {
+                            d -= x[e];
+                            xnew = x[e] + d;  // X .+= Delta (solver_gpu.jl:242)
+                        } else {
+                            xnew = (double)(float)d;  // FP32 storage: Delta of the stored values
+                            d = xnew - x[e];
+                        }
+                        xn[e] = xnew;
+                        rs[e] += d;
+                        csl += d;
+                        dd += d * d;
+                        // avg = (avg + x*nw)*aw (saddle_point_gpu.jl:129-130)
+                        if (!avg_empty) {
+                            double t = xa[e] + xnew * nw;
+                            xan[e] = t * aw;
+                        }
+                    }
+                    if (do_step) {
+                        store_vec<T>(Xn + idx, xn);
+                        if (!avg_empty) store_vec<T>(Xan + idx, xan);
+                    }
+                }
+                const double s = wave_sum(csl);
+                if (colok && lane == 0) colbuf[w][j - jc] = s;
+            }
+        }
+        __syncthreads();
+        if (do_step && tid < jend - jc) {
+            const double t = ((colbuf[0][tid] + colbuf[1][tid]) + colbuf[2][tid]) + colbuf[3][tid];
+            p.colpart[(int64_t)rb * m + jc + tid] = t;
+        }
+        __syncthreads();
+    }
+
+    if (do_step) {
+        if (lane_active) {
+#pragma unroll
+            for (int e = 0; e < VEC; ++e) p.rowpart[(int64_t)cb * ld + i0 + e] = rs[e];
+        }
+        const double ddb = block_sum(dd, red);
+        if (tid == 0) p.sqpart[blockIdx.x] = ddb;
+    }
+    if (EVAL) {
+#pragma unroll
+        for (int k = 0; k < kMatStats; ++k) {
+            const bool is_max = (k == 1 || k == 3 || k == 6 || k == 8);
+            const double t = is_max ? block_max(ev[k], red) : block_sum(ev[k], red);
+            if (tid == 0) p.evalpart[(int64_t)blockIdx.x * kMatStats + k] = t;
+        }
+    }
+}
+
+// ----------------------------------------------------------- reduce kernel
+
+// Finishes one pass: row sums r+ = rowsum(Delta) + r (solver_gpu.jl:172),
+// column sums (:173), the dual update u+ = u + eta*omega*(a - 2 r+ + r)
+// (:191), the averages of u, v, r, c (saddle_point_gpu.jl:140-221) and the
+// vector parts of compute_interaction_and_movement (:272-283).  Thread k of
+// the first NBr blocks owns row k; the next NBc blocks own the columns.  With
+// EVAL it also forms the vector parts of the evaluation of the input point.
+template <bool EVAL>
+__global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
+    __shared__ pdot_solver_state sst;
+    __shared__ int amlast;
+    const int tid = threadIdx.x;
+    if (tid == 0) sst = *p.st;
+    __syncthreads();
+    const bool do_step = !sst.numerical_error;
+    if (!do_step && !EVAL) return;
+    const bool avg_empty = sst.avg_count == 0;
+    const int cs = sst.cur_slot;
+    const int as = avg_empty ? cs : sst.avg_slot;
+    const int s0 = sst.spare0, s1 = sst.spare1;
+    const double eta = sst.step_size, omega = sst.primal_weight, S = sst.sum_weights;
+    double nw = 1.0, aw = 1.0;
+    if (S > 0.0) {
+        aw = S / (S + eta);
+        nw = eta / S;
+    }
+    double dot = 0.0, yy = 0.0;
+    double ev[kVecStats];
+#pragma unroll
+    for (int k = 0; k < kVecStats; ++k) ev[k] = 0.0;
+
+    const bool rows = (int)blockIdx.x < p.NBr;
+    const int64_t k = (int64_t)(rows ? blockIdx.x : blockIdx.x - p.NBr) * kRedBlock + tid;
+    const int64_t len = rows ? p.n : p.m;
+    if (k < len) {
+        const double* part = rows ? p.rowpart : p.colpart;
+        const int64_t pstride = rows ? p.ld : p.m;
+        const int np = rows ? p.Cb : p.R;
+        double* const* ys = rows ? p.s.u : p.s.v;
+        double* const* ss = rows ? p.s.r : p.s.c;
+        const double q = rows ? p.a[k] : p.b[k];
+        const double s_k = ss[cs][k];
+        const double y_k = ys[cs][k];
+        if (do_step) {
+            double sum = 0.0;
+            for (int t = 0; t < np; ++t) sum += part[(int64_t)t * pstride + k];
+            const double sn = sum + s_k;
+            const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
+            const double ds = sn - s_k, dy = yn - y_k;
+            dot = ds * dy;
+            yy = dy * dy;
+            ss[s0][k] = sn;
+            ys[s0][k] = yn;
+            if (!avg_empty) {
+                const double t1 = ss[as][k] + sn * nw;
+                ss[s1][k] = t1 * aw;
+                const double t2 = ys[as][k] + yn * nw;
+                ys[s1][k] = t2 * aw;
+            }
+        }
+        if (EVAL) {
+            const double t = q - s_k;
+            ev[0] = fabs(t);
+            ev[1] = t * t;
+            ev[2] = q * y_k;
+            ev[3] = rows ? fabs(y_k) : 0.0;
+            ev[4] = rows ? y_k * y_k : 0.0;
+            ev[5] = y_k * y_k;
+            if (!avg_empty) {
+                const double sa = ss[as][k], ya = ys[as][k];
+                const double ta = q - sa;
+                ev[6] = fabs(ta);
+                ev[7] = ta * ta;
+                ev[8] = q * ya;
+                ev[9] = rows ? fabs(ya) : 0.0;
+                ev[10] = rows ? ya * ya : 0.0;
+                ev[11] = ya * ya;
+            }
+        }
+    }
+    if (EVAL) {
+#pragma unroll
+        for (int t = 0; t < kVecStats; ++t) {
+            const bool is_max = (t == 0 || t == 3 || t == 6 || t == 9);
+            const double v = is_max ? wave_max(ev[t]) : wave_sum(ev[t]);
+            if (tid == 0) p.evalvec[(int64_t)blockIdx.x * kVecStats + t] = v;
+        }
+    }
+    if (!do_step) return;  // halted: evaluation only, nothing to commit
+    double dx = 0.0;
+    const int64_t lo = (int64_t)blockIdx.x * p.sqchunk;
+    int64_t hi = lo + p.sqchunk;
+    if (hi > p.nstep) hi = p.nstep;
+    for (int64_t t = lo + tid; t < hi; t += kRedBlock) dx += p.sqpart[t];
+    dot = wave_sum(dot);
+    yy = wave_sum(yy);
+    dx = wave_sum(dx);
+    if (tid == 0) {
+        p.redpart[3 * blockIdx.x + 0] = dot;
+        p.redpart[3 * blockIdx.x + 1] = yy;
+        p.redpart[3 * blockIdx.x + 2] = dx;
+        // publish the partials, then count this block as done
+        __threadfence();
+        const unsigned prev = atomicAdd(p.counter, 1u);
+        amlast = (prev == gridDim.x - 1) ? 1 : 0;
+    }
+    __syncthreads();
+    if (!amlast) return;
+    // Last block: every other block has published its partials and read the
+    // state, so the state can be updated in place.
+    __threadfence();
+    if (!EVAL) reduce_and_commit(&sst, p.redpart, p.nred);
+    if (tid == 0) {
+        if (EVAL) sst.pending = 1;  // an evaluation's pass waits for the host's decision
+        *p.st = sst;
+        *p.counter = 0u;
+    }
+}
+
+// Final fixed-order reduction of the evaluation partials into 22 values.
+__global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart, int64_t nstep, const double* evalvec,
+                                                         int64_t nred, double* out) {
+    __shared__ double red[kBlock];
+    const int tid = threadIdx.x;
+    for (int k = 0; k < kMatStats; ++k) {
+        const bool is_max = (k == 1 || k == 3 || k == 6 || k == 8);
+        double acc = 0.0;
+        for (int64_t b = tid; b < nstep; b += kBlock) {
+            const double v = evalpart[b * kMatStats + k];
+            acc = is_max ? nanmax(acc, v) : acc + v;
+        }
+        acc = is_max ? block_max(acc, red) : block_sum(acc, red);
+        if (tid == 0) out[k] = acc;
+    }
+    for (int k = 0; k < kVecStats; ++k) {
+        const bool is_max = (k == 0 || k == 3 || k == 6 || k == 9);
+        double acc = 0.0;
+        for (int64_t b = tid; b < nred; b += kBlock) {
+            const double v = evalvec[b * kVecStats + k];
+            acc = is_max ? nanmax(acc, v) : acc + v;
+        }
+        acc = is_max ? block_max(acc, red) : block_sum(acc, red);
+        if (tid == 0) out[kMatStats + k] = acc;
+    }
+}
+
+// Commits a pending pass (one wave).
+__global__ __launch_bounds__(kRedBlock) void commit_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
+    __shared__ pdot_solver_state sst;
+    if (threadIdx.x == 0) sst = *st;
+    __syncthreads();
+    if (!sst.pending) return;
+    reduce_and_commit(&sst, redpart, nred);
+    if (threadIdx.x == 0) *st = sst;
+}
+
+// FP64 host layout (pitch n) -> device storage (pitch ld), for ncols columns.
+template <typename T>
+__global__ void narrow_kernel(const double* __restrict__ src, T* __restrict__ dst, int64_t n, int64_t ld, int64_t ncols) {
+    const int64_t total = n * ncols;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / n, i = t - j * n;
+        dst[j * ld + i] = (T)src[t];
+    }
+}
+template <typename T>
+__global__ void widen_kernel(const T* __restrict__ src, double* __restrict__ dst, int64_t n, int64_t ld, int64_t ncols) {
+    const int64_t total = n * ncols;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t j = t / n, i = t - j * n;
+        dst[t] = (double)src[j * ld + i];
+    }
+}
+
+}  // namespace
+
+// ====================================================================== host
+
+struct pdot_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t n = 0, m = 0, ld = 0;
+    int32_t storage = PDOT_STORAGE_F64;
+    size_t esz = 8;
+    void* C = nullptr;
+    SlotPtrs s{};
+    double* a = nullptr;
+    double* b = nullptr;
+    double* rowpart = nullptr;
+    double* colpart = nullptr;
+    double* sqpart = nullptr;
+    double* evalpart = nullptr;
+    double* redpart = nullptr;
+    double* evalvec = nullptr;
+    double* evalout = nullptr;
+    double* stage = nullptr;
+    int64_t stage_cols = 0;
+    pdot_solver_state* stA = nullptr;
+    unsigned int* counter = nullptr;      // reduce-kernel completion counter
+    int32_t resident_blocks = 0;          // step-kernel blocks resident at once
+    pdot_solver_state* h_state = nullptr;  // pinned
+    double* h_eval = nullptr;              // pinned
+    int32_t R = 0, Cb = 0, TC = 0, nstep = 0, NBr = 0, NBc = 0, nred = 0, sqchunk = 0;
+    int32_t cb_hint = 0;
+    bool uploaded = false, started = false;
+    bool use_graph = false;
+    std::map<int64_t, hipGraphExec_t> graphs;
+    std::vector<hipEvent_t> events;
+    int64_t bytes = 0;
+    std::string err;
+};
+
+namespace {
+
+int fail(pdot_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, call)                                                                              \
+    do {                                                                                               \
+        hipError_t e_ = (call);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            return fail((ctx), PDOT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_));         \
+        }                                                                                              \
+    } while (0)
+
+int rows_per_block(int32_t storage) { return kWaves * kWave * (storage == PDOT_STORAGE_F64 ? 2 : 4); }
+
+int dev_alloc(pdot_ctx* ctx, void** p, size_t bytes) {
+    if (bytes == 0) bytes = 8;
+    hipError_t e = hipMalloc(p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ctx, PDOT_E_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+    }
+    ctx->bytes += (int64_t)bytes;
+    return PDOT_OK;
+}
+
+void free_graphs(pdot_ctx* ctx) {
+    for (auto& kv : ctx->graphs) hipGraphExecDestroy(kv.second);
+    ctx->graphs.clear();
+}
+
+void free_partials(pdot_ctx* ctx) {
+    double** ps[] = {&ctx->rowpart, &ctx->colpart, &ctx->sqpart, &ctx->evalpart, &ctx->redpart, &ctx->evalvec};
+    for (double** q : ps) {
+        if (*q) hipFree(*q);
+        *q = nullptr;
+    }
+}
+
+// Launch geometry: R row blocks of rows_per_block rows; Cb column blocks.
+int setup_geometry(pdot_ctx* ctx) {
+    free_graphs(ctx);
+    free_partials(ctx);
+    const int TR = rows_per_block(ctx->storage);
+    const int64_t R = (ctx->n + TR - 1) / TR;
+    int64_t Cb;
+    if (ctx->cb_hint > 0) {
+        Cb = ctx->cb_hint;
+    } else {
+        // one full wave of resident blocks: every block streams the same
+        // number of columns and no straggler round follows
+        Cb = ctx->resident_blocks / R;
+    }
+    if (Cb < 1) Cb = 1;
+    if (Cb > ctx->m) Cb = ctx->m;
+    int64_t TC = (ctx->m + Cb - 1) / Cb;
+    Cb = (ctx->m + TC - 1) / TC;
+    if (R * Cb > 0x7fffffffLL) return fail(ctx, PDOT_E_ARG, "grid too large");
+    ctx->R = (int32_t)R;
+    ctx->Cb = (int32_t)Cb;
+    ctx->TC = (int32_t)TC;
+    ctx->nstep = (int32_t)(R * Cb);
+    ctx->NBr = (int32_t)((ctx->n + kRedBlock - 1) / kRedBlock);
+    ctx->NBc = (int32_t)((ctx->m + kRedBlock - 1) / kRedBlock);
+    ctx->nred = ctx->NBr + ctx->NBc;
+    ctx->sqchunk = (ctx->nstep + ctx->nred - 1) / ctx->nred;
+    int rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->rowpart, sizeof(double) * (size_t)(Cb * ctx->ld)))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->colpart, sizeof(double) * (size_t)(R * ctx->m)))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->sqpart, sizeof(double) * (size_t)ctx->nstep))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->evalpart, sizeof(double) * (size_t)ctx->nstep * kMatStats))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->redpart, sizeof(double) * (size_t)ctx->nred * 3))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->evalvec, sizeof(double) * (size_t)ctx->nred * kVecStats))) return rc;
+    return PDOT_OK;
+}
+
+StepParams step_params(pdot_ctx* ctx) {
+    StepParams p;
+    p.C = ctx->C;
+    p.s = ctx->s;
+    p.st = ctx->stA;
+    p.rowpart = ctx->rowpart;
+    p.colpart = ctx->colpart;
+    p.sqpart = ctx->sqpart;
+    p.evalpart = ctx->evalpart;
+    p.n = ctx->n;
+    p.m = ctx->m;
+    p.ld = ctx->ld;
+    p.TC = ctx->TC;
+    p.Cb = ctx->Cb;
+    p.R = ctx->R;
+    return p;
+}
+
+ReduceParams reduce_params(pdot_ctx* ctx) {
+    ReduceParams p;
+    p.s = ctx->s;
+    p.a = ctx->a;
+    p.b = ctx->b;
+    p.st = ctx->stA;
+    p.counter = ctx->counter;
+    p.rowpart = ctx->rowpart;
+    p.colpart = ctx->colpart;
+    p.sqpart = ctx->sqpart;
+    p.redpart = ctx->redpart;
+    p.evalvec = ctx->evalvec;
+    p.n = ctx->n;
+    p.m = ctx->m;
+    p.ld = ctx->ld;
+    p.Cb = ctx->Cb;
+    p.R = ctx->R;
+    p.NBr = ctx->NBr;
+    p.NBc = ctx->NBc;
+    p.nred = ctx->nred;
+    p.nstep = ctx->nstep;
+    p.sqchunk = ctx->sqchunk;
+    return p;
+}
+
+hipError_t launch_step(pdot_ctx* ctx, bool eval) {
+    StepParams p = step_params(ctx);
+    dim3 grid(ctx->nstep), block(kBlock);
+    if (ctx->storage == PDOT_STORAGE_F64) {
+        if (eval) hipLaunchKernelGGL((step_kernel<double, true, kUnroll>), grid, block, 0, ctx->stream, p);
+        else hipLaunchKernelGGL((step_kernel<double, false, kUnroll>), grid, block, 0, ctx->stream, p);
+    } else {
+        if (eval) hipLaunchKernelGGL((step_kernel<float, true, kUnroll>), grid, block, 0, ctx->stream, p);
+        else hipLaunchKernelGGL((step_kernel<float, false, kUnroll>), grid, block, 0, ctx->stream, p);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
+    ReduceParams p = reduce_params(ctx);
+    dim3 grid(ctx->nred), block(kRedBlock);
+    if (eval) hipLaunchKernelGGL((reduce_kernel<true>), grid, block, 0, ctx->stream, p);
+    else hipLaunchKernelGGL((reduce_kernel<false>), grid, block, 0, ctx->stream, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_commit(pdot_ctx* ctx) {
+    hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(kRedBlock), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
+    return hipGetLastError();
+}
+
+// A pass left pending by an evaluation is committed first; each later
+// pass commits itself in its reduce kernel.
+int enqueue_steps(pdot_ctx* ctx, int64_t count) {
+    HIPCHK(ctx, launch_commit(ctx));
+    for (int64_t k = 0; k < count; ++k) {
+        HIPCHK(ctx, launch_step(ctx, false));
+        HIPCHK(ctx, launch_reduce(ctx, false));
+    }
+    return PDOT_OK;
+}
+
+// Host copy of choose_spares.
+void host_choose_spares(pdot_solver_state& st) {
+    int used0 = st.cur_slot, used1 = st.avg_count > 0 ? st.avg_slot : st.cur_slot;
+    int k = 0;
+    for (int s = 0; s < 4; ++s) {
+        if (s == used0 || s == used1) continue;
+        if (k == 0) st.spare0 = s;
+        else if (k == 1) st.spare1 = s;
+        ++k;
+    }
+}
+
+int read_state(pdot_ctx* ctx, pdot_solver_state* out) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_state, ctx->stA, sizeof(pdot_solver_state), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (out) *out = *ctx->h_state;
+    return PDOT_OK;
+}
+
+int write_state(pdot_ctx* ctx, const pdot_solver_state* in) {
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    *ctx->h_state = *in;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->stA, ctx->h_state, sizeof(pdot_solver_state), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PDOT_OK;
+}
+
+int ensure_stage(pdot_ctx* ctx) {
+    if (ctx->stage) return PDOT_OK;
+    const int64_t max_bytes = (int64_t)256 << 20;
+    int64_t cols = max_bytes / (8 * (ctx->n > 0 ? ctx->n : 1));
+    if (cols < 1) cols = 1;
+    if (cols > ctx->m) cols = ctx->m;
+    ctx->stage_cols = cols;
+    return dev_alloc(ctx, (void**)&ctx->stage, sizeof(double) * (size_t)(cols * ctx->n));
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t device) {
+    if (!out) return PDOT_E_ARG;
+    *out = nullptr;
+    if (n <= 0 || m <= 0 || (storage != PDOT_STORAGE_F64 && storage != PDOT_STORAGE_F32)) return PDOT_E_ARG;
+    pdot_ctx* ctx = new pdot_ctx();
+    ctx->n = n;
+    ctx->m = m;
+    ctx->storage = storage;
+    ctx->esz = storage == PDOT_STORAGE_F64 ? 8 : 4;
+    ctx->ld = (n + kLdAlign - 1) / kLdAlign * kLdAlign;
+    ctx->device = device;
+    *out = ctx;
+    HIPCHK(ctx, hipSetDevice(device));
+    HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    const size_t mat = ctx->esz * (size_t)ctx->ld * (size_t)m;
+    int rc;
+    if ((rc = dev_alloc(ctx, &ctx->C, mat))) return rc;
+    for (int s = 0; s < 4; ++s) {
+        if ((rc = dev_alloc(ctx, &ctx->s.X[s], mat))) return rc;
+        if ((rc = dev_alloc(ctx, (void**)&ctx->s.u[s], sizeof(double) * (size_t)ctx->ld))) return rc;
+        if ((rc = dev_alloc(ctx, (void**)&ctx->s.r[s], sizeof(double) * (size_t)ctx->ld))) return rc;
+        if ((rc = dev_alloc(ctx, (void**)&ctx->s.v[s], sizeof(double) * (size_t)m))) return rc;
+        if ((rc = dev_alloc(ctx, (void**)&ctx->s.c[s], sizeof(double) * (size_t)m))) return rc;
+        HIPCHK(ctx, hipMemsetAsync(ctx->s.X[s], 0, mat, ctx->stream));
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->C, 0, mat, ctx->stream));
+    if ((rc = dev_alloc(ctx, (void**)&ctx->a, sizeof(double) * (size_t)ctx->ld))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->b, sizeof(double) * (size_t)m))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->evalout, sizeof(double) * kEvalOut))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->stA, sizeof(pdot_solver_state)))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->counter, sizeof(unsigned int)))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(unsigned int), ctx->stream));
+    HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_state, sizeof(pdot_solver_state), hipHostMallocDefault));
+    HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_eval, sizeof(double) * kEvalOut, hipHostMallocDefault));
+    int ncu = 0, per_cu = 0;
+    HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
+    if (storage == PDOT_STORAGE_F64)
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kUnroll>, kBlock, 0));
+    else
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kUnroll>, kBlock, 0));
+    ctx->resident_blocks = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
+    if ((rc = setup_geometry(ctx))) return rc;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PDOT_OK;
+}
+
+void pdot_destroy(pdot_ctx* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    free_graphs(ctx);
+    for (hipEvent_t e : ctx->events) hipEventDestroy(e);
+    free_partials(ctx);
+    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->counter, ctx->stage};
+    for (void* q : ps)
+        if (q) hipFree(q);
+    for (int s = 0; s < 4; ++s) {
+        void* qs[] = {ctx->s.X[s], ctx->s.u[s], ctx->s.v[s], ctx->s.r[s], ctx->s.c[s]};
+        for (void* q : qs)
+            if (q) hipFree(q);
+    }
+    if (ctx->h_state) hipHostFree(ctx->h_state);
+    if (ctx->h_eval) hipHostFree(ctx->h_eval);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* pdot_last_error(const pdot_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!C || !a || !b) return fail(ctx, PDOT_E_ARG, "pdot_upload: null pointer");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int64_t n = ctx->n, m = ctx->m, ld = ctx->ld;
+    if (ctx->storage == PDOT_STORAGE_F64) {
+        HIPCHK(ctx, hipMemcpy2DAsync(ctx->C, ld * 8, C, n * 8, n * 8, m, hipMemcpyHostToDevice, ctx->stream));
+    } else {
+        int rc = ensure_stage(ctx);
+        if (rc) return rc;
+        for (int64_t j0 = 0; j0 < m; j0 += ctx->stage_cols) {
+            const int64_t nc = (j0 + ctx->stage_cols < m) ? ctx->stage_cols : m - j0;
+            HIPCHK(ctx, hipMemcpyAsync(ctx->stage, C + j0 * n, sizeof(double) * n * nc, hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(narrow_kernel<float>, dim3(2048), dim3(256), 0, ctx->stream, ctx->stage,
+                               static_cast<float*>(ctx->C) + j0 * ld, n, ld, nc);
+            HIPCHK(ctx, hipGetLastError());
+        }
+    }
+    HIPCHK(ctx, hipMemsetAsync(ctx->a, 0, sizeof(double) * ld, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->a, a, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->b, b, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->uploaded = true;
+    return PDOT_OK;
+}
+
+int pdot_set_start(pdot_ctx* ctx, const double* u0, const double* v0, int32_t step_policy, double reduction_exponent,
+                   double growth_exponent, double step_size, double primal_weight) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!ctx->uploaded) return fail(ctx, PDOT_E_STATE, "pdot_set_start before pdot_upload");
+    if ((u0 == nullptr) != (v0 == nullptr)) return fail(ctx, PDOT_E_ARG, "pdot_set_start: give both u0 and v0 or neither");
+    if (step_policy != PDOT_POLICY_ADAPTIVE && step_policy != PDOT_POLICY_CONSTANT)
+        return fail(ctx, PDOT_E_ARG, "pdot_set_start: unknown step policy");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t mat = ctx->esz * (size_t)ctx->ld * (size_t)ctx->m;
+    for (int s = 0; s < 4; ++s) {
+        HIPCHK(ctx, hipMemsetAsync(ctx->s.X[s], 0, mat, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->s.u[s], 0, sizeof(double) * ctx->ld, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->s.r[s], 0, sizeof(double) * ctx->ld, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->s.v[s], 0, sizeof(double) * ctx->m, ctx->stream));
+        HIPCHK(ctx, hipMemsetAsync(ctx->s.c[s], 0, sizeof(double) * ctx->m, ctx->stream));
+    }
+    if (u0) {
+        HIPCHK(ctx, hipMemcpyAsync(ctx->s.u[0], u0, sizeof(double) * ctx->n, hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(ctx->s.v[0], v0, sizeof(double) * ctx->m, hipMemcpyHostToDevice, ctx->stream));
+    }
+    pdot_solver_state st;
+    memset(&st, 0, sizeof(st));
+    st.step_size = step_size;
+    st.primal_weight = primal_weight;
+    st.reduction_exponent = reduction_exponent;
+    st.growth_exponent = growth_exponent;
+    st.step_policy = step_policy;
+    st.gzero = u0 ? 0 : 1;
+    st.cur_slot = 0;
+    st.avg_slot = 0;
+    st.spare0 = 1;
+    st.spare1 = 2;
+    int rc = write_state(ctx, &st);
+    if (rc) return rc;
+    ctx->started = true;
+    return PDOT_OK;
+}
+
+int pdot_run_steps(pdot_ctx* ctx, int64_t count, pdot_solver_state* out) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_run_steps before pdot_set_start");
+    if (count < 0) return fail(ctx, PDOT_E_ARG, "pdot_run_steps: negative count");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->use_graph && count > 1) {
+        auto it = ctx->graphs.find(count);
+        if (it == ctx->graphs.end()) {
+            hipGraph_t g;
+            HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int rc = enqueue_steps(ctx, count);
+            hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (rc) return rc;
+            if (e != hipSuccess) return fail(ctx, PDOT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+            hipGraphExec_t ge;
+            e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (e != hipSuccess) return fail(ctx, PDOT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+            it = ctx->graphs.emplace(count, ge).first;
+        }
+        HIPCHK(ctx, hipGraphLaunch(it->second, ctx->stream));
+    } else {
+        int rc = enqueue_steps(ctx, count);
+        if (rc) return rc;
+    }
+    return read_state(ctx, out);
+}
+
+int pdot_eval(pdot_ctx* ctx, pdot_point_stats* current, pdot_point_stats* average, pdot_solver_state* state) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_eval before pdot_set_start");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, launch_commit(ctx));
+    HIPCHK(ctx, launch_step(ctx, true));
+    HIPCHK(ctx, launch_reduce(ctx, true));
+    hipLaunchKernelGGL(evalfin_kernel, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->evalpart, (int64_t)ctx->nstep,
+                       ctx->evalvec, (int64_t)ctx->nred, ctx->evalout);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipMemcpyAsync(ctx->h_eval, ctx->evalout, sizeof(double) * kEvalOut, hipMemcpyDeviceToHost, ctx->stream));
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    const double* e = ctx->h_eval;
+    const double* v = e + kMatStats;
+    if (current) {
+        current->primal_objective = e[0];
+        current->x_inf = e[1];
+        current->x_sq = e[2];
+        current->rc_inf = e[3];
+        current->rc_sq = e[4];
+        current->p_inf = v[0];
+        current->p_sq = v[1];
+        current->dual_objective = v[2];
+        current->u_inf = v[3];
+        current->u_sq = v[4];
+        current->y_sq = v[5];
+    }
+    if (average) {
+        if (st.avg_count == 0) {
+            memset(average, 0, sizeof(*average));
+        } else {
+            average->primal_objective = e[5];
+            average->x_inf = e[6];
+            average->x_sq = e[7];
+            average->rc_inf = e[8];
+            average->rc_sq = e[9];
+            average->p_inf = v[6];
+            average->p_sq = v[7];
+            average->dual_objective = v[8];
+            average->u_inf = v[9];
+            average->u_sq = v[10];
+            average->y_sq = v[11];
+        }
+    }
+    if (state) *state = st;
+    return PDOT_OK;
+}
+
+int pdot_discard(pdot_ctx* ctx) {
+    if (!ctx) return PDOT_E_ARG;
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    if (!st.pending) return PDOT_OK;
+    st.pending = 0;
+    return write_state(ctx, &st);
+}
+
+int pdot_restart(pdot_ctx* ctx, int32_t to_average, double new_primal_weight) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_restart before pdot_set_start");
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    st.pending = 0;
+    if (to_average) {
+        if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_restart: restart to an empty average");
+        st.cur_slot = st.avg_slot;
+        st.gzero = 0;
+    }
+    st.avg_count = 0;
+    st.sum_weights = 0.0;
+    st.avg_slot = st.cur_slot;
+    st.primal_weight = new_primal_weight;
+    host_choose_spares(st);
+    return write_state(ctx, &st);
+}
+
+int pdot_get_state(pdot_ctx* ctx, pdot_solver_state* out) {
+    if (!ctx || !out) return PDOT_E_ARG;
+    return read_state(ctx, out);
+}
+
+int pdot_set_state(pdot_ctx* ctx, const pdot_solver_state* in) {
+    if (!ctx || !in) return PDOT_E_ARG;
+    if (in->cur_slot < 0 || in->cur_slot > 3 || in->avg_slot < 0 || in->avg_slot > 3 || in->spare0 < 0 ||
+        in->spare0 > 3 || in->spare1 < 0 || in->spare1 > 3)
+        return fail(ctx, PDOT_E_ARG, "pdot_set_state: slot index out of range");
+    return write_state(ctx, in);
+}
+
+int pdot_download(pdot_ctx* ctx, int32_t which, double* X, double* u, double* v) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_download before pdot_set_start");
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    int slot;
+    if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
+    else if (which == PDOT_POINT_AVERAGE) {
+        if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_download: the average is empty");
+        slot = st.avg_slot;
+    } else return fail(ctx, PDOT_E_ARG, "pdot_download: unknown point");
+    const int64_t n = ctx->n, m = ctx->m, ld = ctx->ld;
+    if (X) {
+        if (ctx->storage == PDOT_STORAGE_F64) {
+            HIPCHK(ctx, hipMemcpy2DAsync(X, n * 8, ctx->s.X[slot], ld * 8, n * 8, m, hipMemcpyDeviceToHost, ctx->stream));
+        } else {
+            if ((rc = ensure_stage(ctx))) return rc;
+            for (int64_t j0 = 0; j0 < m; j0 += ctx->stage_cols) {
+                const int64_t nc = (j0 + ctx->stage_cols < m) ? ctx->stage_cols : m - j0;
+                hipLaunchKernelGGL(widen_kernel<float>, dim3(2048), dim3(256), 0, ctx->stream,
+                                   static_cast<const float*>(ctx->s.X[slot]) + j0 * ld, ctx->stage, n, ld, nc);
+                HIPCHK(ctx, hipGetLastError());
+                HIPCHK(ctx, hipMemcpyAsync(X + j0 * n, ctx->stage, sizeof(double) * n * nc, hipMemcpyDeviceToHost, ctx->stream));
+            }
+        }
+    }
+    if (u) HIPCHK(ctx, hipMemcpyAsync(u, ctx->s.u[slot], sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    if (v) HIPCHK(ctx, hipMemcpyAsync(v, ctx->s.v[slot], sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return PDOT_OK;
+}
+
+int pdot_time_steps(pdot_ctx* ctx, int64_t count, double* step_kernel_ms, double* total_ms) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_time_steps before pdot_set_start");
+    if (count < 1) return fail(ctx, PDOT_E_ARG, "pdot_time_steps: count < 1");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const size_t need = (size_t)(2 * count + 2);
+    while (ctx->events.size() < need) {
+        hipEvent_t e;
+        HIPCHK(ctx, hipEventCreate(&e));
+        ctx->events.push_back(e);
+    }
+    hipEvent_t* ev = ctx->events.data();
+    HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
+    HIPCHK(ctx, launch_commit(ctx));
+    for (int64_t k = 0; k < count; ++k) {
+        HIPCHK(ctx, hipEventRecord(ev[2 + 2 * k], ctx->stream));
+        HIPCHK(ctx, launch_step(ctx, false));
+        HIPCHK(ctx, hipEventRecord(ev[3 + 2 * k], ctx->stream));
+        HIPCHK(ctx, launch_reduce(ctx, false));
+    }
+    HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(ev[1]));
+    double ks = 0.0;
+    for (int64_t k = 0; k < count; ++k) {
+        float t = 0.f;
+        HIPCHK(ctx, hipEventElapsedTime(&t, ev[2 + 2 * k], ev[3 + 2 * k]));
+        ks += t;
+    }
+    float tt = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&tt, ev[0], ev[1]));
+    if (step_kernel_ms) *step_kernel_ms = ks;
+    if (total_ms) *total_ms = tt;
+    return PDOT_OK;
+}
+
+int pdot_use_graphs(pdot_ctx* ctx, int32_t enable) {
+    if (!ctx) return PDOT_E_ARG;
+    ctx->use_graph = enable != 0;
+    return PDOT_OK;
+}
+
+int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint) {
+    if (!ctx) return PDOT_E_ARG;
+    if (col_blocks_hint < 0) return fail(ctx, PDOT_E_ARG, "pdot_set_launch: negative hint");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    // the pending pass (if any) keeps its partials only while the geometry is unchanged
+    pdot_solver_state st;
+    if (ctx->started) {
+        int rc = read_state(ctx, &st);
+        if (rc) return rc;
+        if (st.pending) return fail(ctx, PDOT_E_STATE, "pdot_set_launch: a pass is pending");
+    }
+    ctx->cb_hint = col_blocks_hint;
+    return setup_geometry(ctx);
+}
+
+int pdot_get_launch(pdot_ctx* ctx, int32_t* rows_per_blk, int32_t* col_blocks, int32_t* cols_per_block, int32_t* blocks) {
+    if (!ctx) return PDOT_E_ARG;
+    if (rows_per_blk) *rows_per_blk = rows_per_block(ctx->storage);
+    if (col_blocks) *col_blocks = ctx->Cb;
+    if (cols_per_block) *cols_per_block = ctx->TC;
+    if (blocks) *blocks = ctx->nstep;
+    return PDOT_OK;
+}
+
+int64_t pdot_device_bytes(const pdot_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+}  // extern "C"

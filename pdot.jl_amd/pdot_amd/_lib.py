@@ -1,0 +1,244 @@
+"""ctypes binding of libpdot_hip.so (declarations in include/pdot.h).
+
+The HIP library is the only compute path: loading fails loudly when the
+shared object is missing or was built for another ABI.  There is no CPU
+fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpdot_hip.so")
+
+PDOT_OK = 0
+PDOT_E_ARG = -1
+PDOT_E_HIP = -2
+PDOT_E_STATE = -3
+PDOT_E_NOMEM = -4
+
+STORAGE_F64 = 0
+STORAGE_F32 = 1
+POLICY_ADAPTIVE = 0
+POLICY_CONSTANT = 1
+POINT_CURRENT = 0
+POINT_AVERAGE = 1
+
+
+class SolverState(ctypes.Structure):
+    """pdot_solver_state (include/pdot.h)."""
+
+    _fields_ = [
+        ("step_size", ctypes.c_double),
+        ("primal_weight", ctypes.c_double),
+        ("cumulative_kkt_passes", ctypes.c_double),
+        ("sum_weights", ctypes.c_double),
+        ("total_number_iterations", ctypes.c_int64),
+        ("avg_count", ctypes.c_int64),
+        ("take_step_calls", ctypes.c_int64),
+        ("error_call", ctypes.c_int64),
+        ("last_interaction", ctypes.c_double),
+        ("last_movement", ctypes.c_double),
+        ("last_step_limit", ctypes.c_double),
+        ("last_dx2", ctypes.c_double),
+        ("last_dot", ctypes.c_double),
+        ("last_dy2", ctypes.c_double),
+        ("reduction_exponent", ctypes.c_double),
+        ("growth_exponent", ctypes.c_double),
+        ("step_policy", ctypes.c_int32),
+        ("numerical_error", ctypes.c_int32),
+        ("gzero", ctypes.c_int32),
+        ("cur_slot", ctypes.c_int32),
+        ("avg_slot", ctypes.c_int32),
+        ("pending", ctypes.c_int32),
+        ("spare0", ctypes.c_int32),
+        ("spare1", ctypes.c_int32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class PointStats(ctypes.Structure):
+    """pdot_point_stats (include/pdot.h)."""
+
+    _fields_ = [
+        ("primal_objective", ctypes.c_double),
+        ("x_inf", ctypes.c_double),
+        ("x_sq", ctypes.c_double),
+        ("rc_inf", ctypes.c_double),
+        ("rc_sq", ctypes.c_double),
+        ("p_inf", ctypes.c_double),
+        ("p_sq", ctypes.c_double),
+        ("dual_objective", ctypes.c_double),
+        ("u_inf", ctypes.c_double),
+        ("u_sq", ctypes.c_double),
+        ("y_sq", ctypes.c_double),
+    ]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_c_double_p = ctypes.POINTER(ctypes.c_double)
+_ctx_p = ctypes.c_void_p
+
+# name -> (restype, argtypes); the exported C-ABI surface
+SIGNATURES = {
+    "pdot_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),
+    "pdot_destroy": (None, [_ctx_p]),
+    "pdot_last_error": (ctypes.c_char_p, [_ctx_p]),
+    "pdot_upload": (ctypes.c_int, [_ctx_p, _c_double_p, _c_double_p, _c_double_p]),
+    "pdot_set_start": (ctypes.c_int, [_ctx_p, _c_double_p, _c_double_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "pdot_run_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, ctypes.POINTER(SolverState)]),
+    "pdot_eval": (ctypes.c_int, [_ctx_p, ctypes.POINTER(PointStats), ctypes.POINTER(PointStats), ctypes.POINTER(SolverState)]),
+    "pdot_discard": (ctypes.c_int, [_ctx_p]),
+    "pdot_restart": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_double]),
+    "pdot_get_state": (ctypes.c_int, [_ctx_p, ctypes.POINTER(SolverState)]),
+    "pdot_set_state": (ctypes.c_int, [_ctx_p, ctypes.POINTER(SolverState)]),
+    "pdot_download": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p]),
+    "pdot_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
+    "pdot_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
+}
+
+_lib = None
+
+
+def load_library(path: str | None = None) -> ctypes.CDLL:
+    """Load libpdot_hip.so; raises if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(
+            f"libpdot_hip.so not found at {p}: build it first (python pdot.jl_amd/build.py); "
+            "the PDOT hot path has no CPU fallback")
+    lib = ctypes.CDLL(p)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(_c_double_p)
+
+
+class PdotError(RuntimeError):
+    pass
+
+
+class DeviceSolver:
+    """One pdot_ctx: a problem resident in HBM plus its solver state."""
+
+    def __init__(self, n: int, m: int, storage: int = STORAGE_F64, device: int = 0):
+        self.lib = load_library()
+        self.n, self.m, self.storage = int(n), int(m), int(storage)
+        h = ctypes.c_void_p()
+        rc = self.lib.pdot_create(ctypes.byref(h), self.n, self.m, self.storage, int(device))
+        self.h = h
+        if rc != PDOT_OK:
+            msg = self.lib.pdot_last_error(h).decode() if h.value else f"code {rc}"
+            if h.value:
+                self.lib.pdot_destroy(h)
+            self.h = ctypes.c_void_p()
+            raise PdotError(f"pdot_create failed ({rc}): {msg}")
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != PDOT_OK:
+            raise PdotError(f"{what} failed ({rc}): {self.lib.pdot_last_error(self.h).decode()}")
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            self.lib.pdot_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, C: np.ndarray, a: np.ndarray, b: np.ndarray) -> None:
+        C = np.asfortranarray(C, dtype=np.float64)
+        if C.shape != (self.n, self.m):
+            raise ValueError(f"cost matrix shape {C.shape} != {(self.n, self.m)}")
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        if a.shape != (self.n,) or b.shape != (self.m,):
+            raise ValueError("distribution lengths do not match the cost matrix")
+        self._check(self.lib.pdot_upload(self.h, C.ctypes.data_as(_c_double_p), _ptr(a), _ptr(b)), "pdot_upload")
+
+    def set_start(self, u0=None, v0=None, policy=POLICY_ADAPTIVE, reduction_exponent=2.0,
+                  growth_exponent=1.0, step_size=1.0, primal_weight=1.0) -> None:
+        if (u0 is None) != (v0 is None):
+            raise ValueError("give both u0 and v0 or neither")
+        if u0 is not None:
+            u0 = np.ascontiguousarray(u0, dtype=np.float64)
+            v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        self._check(self.lib.pdot_set_start(self.h, _ptr(u0), _ptr(v0), int(policy), float(reduction_exponent),
+                                             float(growth_exponent), float(step_size), float(primal_weight)),
+                    "pdot_set_start")
+
+    def run_steps(self, count: int) -> SolverState:
+        st = SolverState()
+        self._check(self.lib.pdot_run_steps(self.h, int(count), ctypes.byref(st)), "pdot_run_steps")
+        return st
+
+    def eval(self):
+        cur, avg, st = PointStats(), PointStats(), SolverState()
+        self._check(self.lib.pdot_eval(self.h, ctypes.byref(cur), ctypes.byref(avg), ctypes.byref(st)), "pdot_eval")
+        return cur, avg, st
+
+    def discard(self) -> None:
+        self._check(self.lib.pdot_discard(self.h), "pdot_discard")
+
+    def restart(self, to_average: bool, primal_weight: float) -> None:
+        self._check(self.lib.pdot_restart(self.h, int(bool(to_average)), float(primal_weight)), "pdot_restart")
+
+    def get_state(self) -> SolverState:
+        st = SolverState()
+        self._check(self.lib.pdot_get_state(self.h, ctypes.byref(st)), "pdot_get_state")
+        return st
+
+    def set_state(self, st: SolverState) -> None:
+        self._check(self.lib.pdot_set_state(self.h, ctypes.byref(st)), "pdot_set_state")
+
+    def download(self, which: int = POINT_CURRENT, want_x: bool = True):
+        X = np.empty((self.n, self.m), dtype=np.float64, order="F") if want_x else None
+        u = np.empty(self.n, dtype=np.float64)
+        v = np.empty(self.m, dtype=np.float64)
+        xp = X.ctypes.data_as(_c_double_p) if X is not None else None
+        self._check(self.lib.pdot_download(self.h, int(which), xp, _ptr(u), _ptr(v)), "pdot_download")
+        return X, u, v
+
+    def time_steps(self, count: int):
+        ks, tt = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.pdot_time_steps(self.h, int(count), ctypes.byref(ks), ctypes.byref(tt)), "pdot_time_steps")
+        return ks.value, tt.value
+
+    def use_graphs(self, enable: bool) -> None:
+        self._check(self.lib.pdot_use_graphs(self.h, int(bool(enable))), "pdot_use_graphs")
+
+    def set_launch(self, col_blocks_hint: int) -> None:
+        self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint)), "pdot_set_launch")
+
+    def get_launch(self) -> dict:
+        vals = [ctypes.c_int32() for _ in range(4)]
+        self._check(self.lib.pdot_get_launch(self.h, *[ctypes.byref(v) for v in vals]), "pdot_get_launch")
+        return dict(zip(("rows_per_block", "col_blocks", "cols_per_block", "blocks"), (v.value for v in vals)))
+
+    def device_bytes(self) -> int:
+        return int(self.lib.pdot_device_bytes(self.h))
