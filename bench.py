@@ -1,0 +1,174 @@
+"""Benchmark of the PDOT hot path on one MI355X.
+
+Workload: the 10000 x 10000 Gaussian-shift transport problem (source N(0,I2),
+target N((2,0),I2), squared Euclidean cost / max, U(0,1) masses, seeded
+synthetic data), solved by restarted adaptive PDHG.
+
+A step is one PDHG iteration (one take_step! pass over C, X and the running
+average).  The timed region runs K steps of the FP64 form (the reference's
+precision) on inputs resident in HBM, with HIP events recorded on the
+library's stream around every launch of the streaming step kernel, so the
+same run gives the iteration rate and the step kernel's average duration.
+
+Also reported (extra fields): the same measurement for the FP32-storage form,
+full solves to 1e-4 relative KKT for both forms (time from after the upload
+to the terminating evaluation, objectives against each other), and the CPU
+baseline: the oracle's literal restatement of the reference on the same
+instance, a bounded number of iterations on the host cores.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "pdot.jl_amd"))
+
+METRIC = "time-to-1e-4 rel KKT at 10k×10k, FP32/learned vs FP64; iters/s; HBM GB/s %"
+PEAK_HBM_GBS = 8000.0
+
+
+def algorithmic_bytes_per_step(n: int, m: int, elem: int) -> int:
+    """Bytes one PDHG iteration must move: read C, X, X_avg and write X+,
+    X_avg+ (5 n*m element passes), plus the row/column vectors (u, v, r, c
+    and their averages read and written, a and b read: about 10 doubles per
+    row and per column)."""
+    return 5 * n * m * elem + 10 * 8 * (n + m)
+
+
+def measure_steps(dev, steps: int, warmup: int):
+    from pdot_amd import _lib
+    dev.set_start(policy=_lib.POLICY_ADAPTIVE, step_size=1.0, primal_weight=1.0)
+    if warmup:
+        dev.time_steps(warmup)
+    ks, tt = dev.time_steps(steps)
+    st = dev.get_state()
+    return ks, tt, st
+
+
+def cpu_baseline(prob, iters: int):
+    """The reference algorithm restated on host arrays (oracle literal mode:
+    stores G and its average like the reference, 8 n*m FP64 arrays),
+    OpenMP across the host cores, timed inside take_step! only."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    import pdot_amd as P
+    params = P.default_parameters(1e-4, iteration_limit=iters + 1)
+    t0 = time.time()
+    r = O.optimize(params, prob, mode=O.LITERAL)
+    wall = time.time() - t0
+    st = r["stats"][-1]
+    steps = int(r["take_step_calls"])
+    basic = st["time_spent_doing_basic_algorithm"]
+    return {"value": steps / basic, "unit": "iters/s", "cores": O.num_threads(), "kind": "port",
+            "sample": f"{steps} take_step! iterations of the literal FP64 restatement on the same "
+                      f"{prob.num_row}x{prob.num_col} instance (evaluations excluded; wall {wall:.1f}s incl. "
+                      f"{len(r['stats'])} evaluations)"}
+
+
+def solve_to_tol(prob, storage, tol, dev=None):
+    import pdot_amd as P
+    prof = P.SolveProfile()
+    out = P.optimize(P.default_parameters(tol, verbosity=0, record_iteration_stats=False), prob,
+                     storage=storage, solver=dev, profile=prof)
+    ci = out.iteration_stats[-1].convergence_information[0]
+    return {"seconds": prof.solve_sec, "iterations": out.iteration_count,
+            "termination": out.termination_string, "objective": ci.primal_objective,
+            "rel_kkt": P.relative_kkt_error(ci), "evaluations": prof.evaluations, "restarts": prof.restarts}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=10000)
+    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+              file=sys.stderr)
+        return 2
+
+    from pdot_amd import _lib, instances
+
+    n = m = args.n
+    prob = instances.gaussian_shift(n)
+    results = {}
+    devs = {}
+    for storage, name, elem in ((_lib.STORAGE_F64, "f64", 8), (_lib.STORAGE_F32, "f32", 4)):
+        dev = _lib.DeviceSolver(n, m, storage=storage)
+        dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+        ks, tt, st = measure_steps(dev, args.steps, args.warmup)
+        alg = algorithmic_bytes_per_step(n, m, elem)
+        k_avg = ks / args.steps * 1e-3
+        results[name] = {
+            "ms_per_step": tt / args.steps, "iters_per_s": args.steps / (tt * 1e-3),
+            "step_kernel_us": k_avg * 1e6, "step_kernel_GBs": alg / k_avg / 1e9,
+            "iteration_GBs": alg / (tt / args.steps * 1e-3) / 1e9, "bytes_per_step": alg,
+            "launch": dev.get_launch(), "numerical_error": int(st.numerical_error),
+        }
+        devs[name] = dev
+
+    solves = {}
+    if not args.no_solve:
+        for name, storage in (("f64", _lib.STORAGE_F64), ("f32", _lib.STORAGE_F32)):
+            solves[name] = solve_to_tol(prob, storage, 1e-4, dev=devs[name])
+        solves["speedup_f32_vs_f64"] = solves["f64"]["seconds"] / solves["f32"]["seconds"]
+        solves["objective_rel_diff_f32_vs_f64"] = (abs(solves["f32"]["objective"] - solves["f64"]["objective"])
+                                                   / abs(solves["f64"]["objective"]))
+    copy_probe = None
+    if args.copy_probe_mib:
+        nbytes = args.copy_probe_mib << 20
+        ms = devs["f64"].copy_probe(nbytes, reps=5)
+        copy_probe = {"bytes_per_copy": 2 * nbytes, "ms": ms, "GBs": 2 * nbytes / (ms * 1e-3) / 1e9,
+                      "launches": 6}
+    for d in devs.values():
+        d.close()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(prob, args.cpu_iters)
+
+    f64 = results["f64"]
+    line = {
+        "metric": METRIC,
+        "value": f64["iters_per_s"],
+        "unit": "iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f64["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"gaussian_shift {n}x{m} FP64 (squared Euclidean / max, U(0,1) masses, seed 2000)",
+                   "n": n, "m": m, "storage": "f64", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": f64["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": f64["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "step_kernel<double>", "bytes_per_launch": f64["bytes_per_step"],
+                     "avg_launch_us": f64["step_kernel_us"]},
+        "cpu_baseline": cpu,
+        "iteration_GBs": f64["iteration_GBs"],
+        "f32_storage": results["f32"],
+        "time_to_1e-4": solves or None,
+    }
+    if copy_probe:
+        line["copy_probe"] = copy_probe
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -154,13 +154,22 @@ int pdot_use_graphs(pdot_ctx* ctx, int32_t enable);
 
 /* Launch geometry of the streaming step kernel: rows per block, column
  * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the
- * default choice (used by the launch-shape tuner); 0 keeps the default. */
-int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint);
+ * default choice (used by the launch-shape tuner); 0 keeps the default.
+ * interleave = 1 (the default) makes column block cb take columns cb,
+ * cb + Cb, ... instead of a contiguous range. */
+int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint, int32_t interleave);
 int pdot_get_launch(pdot_ctx* ctx, int32_t* rows_per_block, int32_t* col_blocks, int32_t* cols_per_block,
                     int32_t* blocks);
 
 /* Bytes of device memory held by the context. */
 int64_t pdot_device_bytes(const pdot_ctx* ctx);
+
+/* Measurement hook with no counterpart in the reference: copies `bytes`
+ * (a multiple of 16) between two scratch buffers with a plain 16-byte copy
+ * kernel, `reps` times after one warm-up, and returns the mean time per copy.
+ * Used to calibrate HBM byte counters and bandwidth against a kernel of known
+ * traffic (2 * bytes per copy). */
+int pdot_copy_probe(pdot_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy);
 
 #ifdef __cplusplus
 }

@@ -47,7 +47,8 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kWaves = 4;
 constexpr int kBlock = kWave * kWaves;
-constexpr int kRedBlock = kWave;  // reduce kernel: one wave per block
+constexpr int kRedWaves = 4;                 // reduce kernel: waves per block
+constexpr int kRedBlock = kWave * kRedWaves;  // each block owns 64 rows or columns
 constexpr int kChunk = 64;        // columns per LDS flush of column sums
 constexpr int kLdAlign = 64;      // leading-dimension alignment (elements)
 constexpr int kMatStats = 10;     // per-block matrix eval reductions
@@ -76,6 +77,7 @@ struct StepParams {
     double* evalpart;
     int64_t n, m, ld;
     int32_t TC, Cb, R;
+    int32_t interleave;  // 1: block cb takes columns cb, cb+Cb, ...; 0: a contiguous range
 };
 
 struct ReduceParams {
@@ -271,7 +273,8 @@ __device__ void controller_commit(pdot_solver_state& st, double dot, double yy, 
 }
 
 // Reduce the per-block scalars of the reduce kernel (dot, ||dy||^2, ||dX||^2
-// chunk) in a fixed order and commit.  Called by all 64 lanes of one wave.
+// chunk) in a fixed order and commit.  Called by the 64 lanes of one wave;
+// lane 0 updates *sst.
 __device__ void reduce_and_commit(pdot_solver_state* sst, const double* redpart, int nred) {
     const int tid = threadIdx.x;
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
@@ -284,30 +287,48 @@ __device__ void reduce_and_commit(pdot_solver_state* sst, const double* redpart,
     d1 = wave_sum(d1);
     d2 = wave_sum(d2);
     if (tid == 0) controller_commit(*sst, d0, d1, d2);
-    __syncthreads();
 }
 
 // ---------------------------------------------------------------- vectors
 
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
 template <typename T> struct VecT;
 template <> struct VecT<double> {
     static constexpr int N = 2;
-    using type = double2;
+    using type = dbl2;
 };
 template <> struct VecT<float> {
     static constexpr int N = 4;
-    using type = float4;
+    using type = flt4;
 };
+
+#ifndef PDOT_NT
+#define PDOT_NT 1
+#endif
+// 16-byte loads/stores of the streamed matrices; with PDOT_NT the accesses
+// carry the nontemporal hint (the matrices are touched once per pass).
+template <typename V>
+__device__ __forceinline__ V ld16(const V* p) {
+    if constexpr (PDOT_NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename V>
+__device__ __forceinline__ void st16(V v, V* p) {
+    if constexpr (PDOT_NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 template <typename V, typename T>
 __device__ __forceinline__ void unpack(const V& v, double* out);
 template <>
-__device__ __forceinline__ void unpack<double2, double>(const double2& v, double* o) {
+__device__ __forceinline__ void unpack<dbl2, double>(const dbl2& v, double* o) {
     o[0] = v.x;
     o[1] = v.y;
 }
 template <>
-__device__ __forceinline__ void unpack<float4, float>(const float4& v, double* o) {
+__device__ __forceinline__ void unpack<flt4, float>(const flt4& v, double* o) {
     o[0] = (double)v.x;
     o[1] = (double)v.y;
     o[2] = (double)v.z;
@@ -318,11 +339,13 @@ template <typename T>
 __device__ __forceinline__ void store_vec(T* p, const double* o);
 template <>
 __device__ __forceinline__ void store_vec<double>(double* p, const double* o) {
-    *reinterpret_cast<double2*>(p) = make_double2(o[0], o[1]);
+    dbl2 v = {o[0], o[1]};
+    st16(v, reinterpret_cast<dbl2*>(p));
 }
 template <>
 __device__ __forceinline__ void store_vec<float>(float* p, const double* o) {
-    *reinterpret_cast<float4*>(p) = make_float4((float)o[0], (float)o[1], (float)o[2], (float)o[3]);
+    flt4 v = {(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
+    st16(v, reinterpret_cast<flt4*>(p));
 }
 
 // ------------------------------------------------------------- step kernel
@@ -358,8 +381,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
     const int64_t n = p.n, m = p.m, ld = p.ld;
     const int64_t i0 = (int64_t)rb * TR + (int64_t)w * WROWS + (int64_t)lane * VEC;
     const bool lane_active = i0 < n;
-    const int64_t j0 = (int64_t)cb * p.TC;
-    const int64_t j1 = (j0 + p.TC < m) ? j0 + p.TC : m;
+    // the block's columns: j(t) = cb + t*Cb (interleaved) or j0 + t, t < ncols
+    const bool ilv = p.interleave != 0;
+    const int64_t j0 = ilv ? cb : (int64_t)cb * p.TC;
+    const int64_t jstep = ilv ? p.Cb : 1;
+    int64_t ncols;
+    if (ilv) ncols = (m - cb + p.Cb - 1) / p.Cb;
+    else ncols = ((j0 + p.TC < m) ? j0 + p.TC : m) - j0;
 
     const bool avg_empty = sst.avg_count == 0;
     const int cs = sst.cur_slot;
@@ -399,28 +427,28 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
 #pragma unroll
     for (int k = 0; k < kMatStats; ++k) ev[k] = 0.0;
 
-    for (int64_t jc = j0; jc < j1; jc += kChunk) {
-        const int64_t jend = (jc + kChunk < j1) ? jc + kChunk : j1;
-        for (int64_t jb = jc; jb < jend; jb += UNROLL) {
+    for (int64_t tc = 0; tc < ncols; tc += kChunk) {
+        const int64_t tend = (tc + kChunk < ncols) ? tc + kChunk : ncols;
+        for (int64_t tb = tc; tb < tend; tb += UNROLL) {
             V xv[UNROLL], cv[UNROLL], av[UNROLL];
             double vj[UNROLL], vaj[UNROLL];
 #pragma unroll
             for (int q = 0; q < UNROLL; ++q) {
-                const int64_t j = jb + q;
-                const bool colok = j < jend;
+                const bool colok = tb + q < tend;
+                const int64_t j = j0 + (tb + q) * jstep;
                 vj[q] = colok ? vc[j] : 0.0;
                 vaj[q] = (EVAL && colok) ? va[j] : 0.0;
                 if (colok && lane_active) {
                     const int64_t idx = j * ld + i0;
-                    xv[q] = *reinterpret_cast<const V*>(Xc + idx);
-                    cv[q] = *reinterpret_cast<const V*>(C + idx);
-                    if (!avg_empty) av[q] = *reinterpret_cast<const V*>(Xa + idx);
+                    xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
+                    cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
+                    if (!avg_empty) av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
                 }
             }
 #pragma unroll
             for (int q = 0; q < UNROLL; ++q) {
-                const int64_t j = jb + q;
-                const bool colok = j < jend;
+                const bool colok = tb + q < tend;
+                const int64_t j = j0 + (tb + q) * jstep;
                 double csl = 0.0;
                 if (colok && lane_active) {
                     const int64_t idx = j * ld + i0;
@@ -476,13 +504,13 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
                     }
                 }
                 const double s = wave_sum(csl);
-                if (colok && lane == 0) colbuf[w][j - jc] = s;
+                if (colok && lane == 0) colbuf[w][tb + q - tc] = s;
             }
         }
         __syncthreads();
-        if (do_step && tid < jend - jc) {
+        if (do_step && tid < tend - tc) {
             const double t = ((colbuf[0][tid] + colbuf[1][tid]) + colbuf[2][tid]) + colbuf[3][tid];
-            p.colpart[(int64_t)rb * m + jc + tid] = t;
+            p.colpart[(int64_t)rb * m + j0 + (tc + tid) * jstep] = t;
         }
         __syncthreads();
     }
@@ -517,111 +545,146 @@ template <bool EVAL>
 __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     __shared__ pdot_solver_state sst;
     __shared__ int amlast;
+    __shared__ double psum[kRedWaves][kWave];
     const int tid = threadIdx.x;
-    if (tid == 0) sst = *p.st;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    if (tid == 0) {
+        sst = *p.st;
+        amlast = 0;
+    }
     __syncthreads();
     const bool do_step = !sst.numerical_error;
     if (!do_step && !EVAL) return;
-    const bool avg_empty = sst.avg_count == 0;
-    const int cs = sst.cur_slot;
-    const int as = avg_empty ? cs : sst.avg_slot;
-    const int s0 = sst.spare0, s1 = sst.spare1;
-    const double eta = sst.step_size, omega = sst.primal_weight, S = sst.sum_weights;
-    double nw = 1.0, aw = 1.0;
-    if (S > 0.0) {
-        aw = S / (S + eta);
-        nw = eta / S;
-    }
-    double dot = 0.0, yy = 0.0;
-    double ev[kVecStats];
-#pragma unroll
-    for (int k = 0; k < kVecStats; ++k) ev[k] = 0.0;
 
     const bool rows = (int)blockIdx.x < p.NBr;
-    const int64_t k = (int64_t)(rows ? blockIdx.x : blockIdx.x - p.NBr) * kRedBlock + tid;
+    const int64_t k = (int64_t)(rows ? blockIdx.x : blockIdx.x - p.NBr) * kWave + lane;
     const int64_t len = rows ? p.n : p.m;
-    if (k < len) {
-        const double* part = rows ? p.rowpart : p.colpart;
-        const int64_t pstride = rows ? p.ld : p.m;
-        const int np = rows ? p.Cb : p.R;
-        double* const* ys = rows ? p.s.u : p.s.v;
-        double* const* ss = rows ? p.s.r : p.s.c;
-        const double q = rows ? p.a[k] : p.b[k];
-        const double s_k = ss[cs][k];
-        const double y_k = ys[cs][k];
-        if (do_step) {
-            double sum = 0.0;
-            for (int t = 0; t < np; ++t) sum += part[(int64_t)t * pstride + k];
-            const double sn = sum + s_k;
-            const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
-            const double ds = sn - s_k, dy = yn - y_k;
-            dot = ds * dy;
-            yy = dy * dy;
-            ss[s0][k] = sn;
-            ys[s0][k] = yn;
-            if (!avg_empty) {
-                const double t1 = ss[as][k] + sn * nw;
-                ss[s1][k] = t1 * aw;
-                const double t2 = ys[as][k] + yn * nw;
-                ys[s1][k] = t2 * aw;
+    const double* part = rows ? p.rowpart : p.colpart;
+    const int64_t pstride = rows ? p.ld : p.m;
+    const int np = rows ? p.Cb : p.R;
+
+    // Stage 1: the partials of entry k, split into kRedWaves contiguous ranges
+    // (one per wave), each summed with four independent accumulators so that
+    // several loads are in flight; combined below in a fixed order.
+    if (do_step) {
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        if (k < len) {
+            const int t0 = (int)(((int64_t)np * w) / kRedWaves);
+            const int t1 = (int)(((int64_t)np * (w + 1)) / kRedWaves);
+            int t = t0;
+            for (; t + 4 <= t1; t += 4) {
+                a0 += part[(int64_t)t * pstride + k];
+                a1 += part[(int64_t)(t + 1) * pstride + k];
+                a2 += part[(int64_t)(t + 2) * pstride + k];
+                a3 += part[(int64_t)(t + 3) * pstride + k];
+            }
+            for (; t < t1; ++t) a0 += part[(int64_t)t * pstride + k];
+        }
+        psum[w][lane] = (a0 + a1) + (a2 + a3);
+    }
+    __syncthreads();
+
+    if (w == 0) {
+        const bool avg_empty = sst.avg_count == 0;
+        const int cs = sst.cur_slot;
+        const int as = avg_empty ? cs : sst.avg_slot;
+        const int s0 = sst.spare0, s1 = sst.spare1;
+        const double eta = sst.step_size, omega = sst.primal_weight, S = sst.sum_weights;
+        double nw = 1.0, aw = 1.0;
+        if (S > 0.0) {
+            aw = S / (S + eta);
+            nw = eta / S;
+        }
+        double dot = 0.0, yy = 0.0;
+        double ev[kVecStats];
+#pragma unroll
+        for (int t = 0; t < kVecStats; ++t) ev[t] = 0.0;
+        if (k < len) {
+            double* const* ys = rows ? p.s.u : p.s.v;
+            double* const* ss = rows ? p.s.r : p.s.c;
+            const double q = rows ? p.a[k] : p.b[k];
+            const double s_k = ss[cs][k];
+            const double y_k = ys[cs][k];
+            if (do_step) {
+                double sum = psum[0][lane];
+#pragma unroll
+                for (int t = 1; t < kRedWaves; ++t) sum += psum[t][lane];
+                const double sn = sum + s_k;
+                const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
+                const double ds = sn - s_k, dy = yn - y_k;
+                dot = ds * dy;
+                yy = dy * dy;
+                ss[s0][k] = sn;
+                ys[s0][k] = yn;
+                if (!avg_empty) {
+                    const double t1 = ss[as][k] + sn * nw;
+                    ss[s1][k] = t1 * aw;
+                    const double t2 = ys[as][k] + yn * nw;
+                    ys[s1][k] = t2 * aw;
+                }
+            }
+            if (EVAL) {
+                const double t = q - s_k;
+                ev[0] = fabs(t);
+                ev[1] = t * t;
+                ev[2] = q * y_k;
+                ev[3] = rows ? fabs(y_k) : 0.0;
+                ev[4] = rows ? y_k * y_k : 0.0;
+                ev[5] = y_k * y_k;
+                if (!avg_empty) {
+                    const double sa = ss[as][k], ya = ys[as][k];
+                    const double ta = q - sa;
+                    ev[6] = fabs(ta);
+                    ev[7] = ta * ta;
+                    ev[8] = q * ya;
+                    ev[9] = rows ? fabs(ya) : 0.0;
+                    ev[10] = rows ? ya * ya : 0.0;
+                    ev[11] = ya * ya;
+                }
             }
         }
         if (EVAL) {
-            const double t = q - s_k;
-            ev[0] = fabs(t);
-            ev[1] = t * t;
-            ev[2] = q * y_k;
-            ev[3] = rows ? fabs(y_k) : 0.0;
-            ev[4] = rows ? y_k * y_k : 0.0;
-            ev[5] = y_k * y_k;
-            if (!avg_empty) {
-                const double sa = ss[as][k], ya = ys[as][k];
-                const double ta = q - sa;
-                ev[6] = fabs(ta);
-                ev[7] = ta * ta;
-                ev[8] = q * ya;
-                ev[9] = rows ? fabs(ya) : 0.0;
-                ev[10] = rows ? ya * ya : 0.0;
-                ev[11] = ya * ya;
+#pragma unroll
+            for (int t = 0; t < kVecStats; ++t) {
+                const bool is_max = (t == 0 || t == 3 || t == 6 || t == 9);
+                const double v = is_max ? wave_max(ev[t]) : wave_sum(ev[t]);
+                if (lane == 0) p.evalvec[(int64_t)blockIdx.x * kVecStats + t] = v;
+            }
+        }
+        if (do_step) {
+            double dx = 0.0;
+            const int64_t lo = (int64_t)blockIdx.x * p.sqchunk;
+            int64_t hi = lo + p.sqchunk;
+            if (hi > p.nstep) hi = p.nstep;
+            for (int64_t t = lo + lane; t < hi; t += kWave) dx += p.sqpart[t];
+            dot = wave_sum(dot);
+            yy = wave_sum(yy);
+            dx = wave_sum(dx);
+            if (lane == 0) {
+                p.redpart[3 * blockIdx.x + 0] = dot;
+                p.redpart[3 * blockIdx.x + 1] = yy;
+                p.redpart[3 * blockIdx.x + 2] = dx;
+                // publish the partials, then count this block as done
+                __threadfence();
+                const unsigned prev = atomicAdd(p.counter, 1u);
+                amlast = (prev == gridDim.x - 1) ? 1 : 0;
             }
         }
     }
-    if (EVAL) {
-#pragma unroll
-        for (int t = 0; t < kVecStats; ++t) {
-            const bool is_max = (t == 0 || t == 3 || t == 6 || t == 9);
-            const double v = is_max ? wave_max(ev[t]) : wave_sum(ev[t]);
-            if (tid == 0) p.evalvec[(int64_t)blockIdx.x * kVecStats + t] = v;
-        }
-    }
     if (!do_step) return;  // halted: evaluation only, nothing to commit
-    double dx = 0.0;
-    const int64_t lo = (int64_t)blockIdx.x * p.sqchunk;
-    int64_t hi = lo + p.sqchunk;
-    if (hi > p.nstep) hi = p.nstep;
-    for (int64_t t = lo + tid; t < hi; t += kRedBlock) dx += p.sqpart[t];
-    dot = wave_sum(dot);
-    yy = wave_sum(yy);
-    dx = wave_sum(dx);
-    if (tid == 0) {
-        p.redpart[3 * blockIdx.x + 0] = dot;
-        p.redpart[3 * blockIdx.x + 1] = yy;
-        p.redpart[3 * blockIdx.x + 2] = dx;
-        // publish the partials, then count this block as done
-        __threadfence();
-        const unsigned prev = atomicAdd(p.counter, 1u);
-        amlast = (prev == gridDim.x - 1) ? 1 : 0;
-    }
     __syncthreads();
     if (!amlast) return;
     // Last block: every other block has published its partials and read the
     // state, so the state can be updated in place.
     __threadfence();
-    if (!EVAL) reduce_and_commit(&sst, p.redpart, p.nred);
-    if (tid == 0) {
-        if (EVAL) sst.pending = 1;  // an evaluation's pass waits for the host's decision
-        *p.st = sst;
-        *p.counter = 0u;
+    if (w == 0) {
+        if (!EVAL) reduce_and_commit(&sst, p.redpart, p.nred);
+        if (tid == 0) {
+            if (EVAL) sst.pending = 1;  // an evaluation's pass waits for the host's decision
+            *p.st = sst;
+            *p.counter = 0u;
+        }
     }
 }
 
@@ -653,13 +716,19 @@ __global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart,
 }
 
 // Commits a pending pass (one wave).
-__global__ __launch_bounds__(kRedBlock) void commit_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
+__global__ __launch_bounds__(kWave) void commit_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
     __shared__ pdot_solver_state sst;
     if (threadIdx.x == 0) sst = *st;
     __syncthreads();
     if (!sst.pending) return;
     reduce_and_commit(&sst, redpart, nred);
     if (threadIdx.x == 0) *st = sst;
+}
+
+// Plain 16-byte copy (counter and bandwidth calibration).
+__global__ __launch_bounds__(256) void copy_kernel(const dbl2* __restrict__ src, dbl2* __restrict__ dst, int64_t nv) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < nv) dst[i] = src[i];
 }
 
 // FP64 host layout (pitch n) -> device storage (pitch ld), for ncols columns.
@@ -710,6 +779,7 @@ struct pdot_ctx {
     double* h_eval = nullptr;              // pinned
     int32_t R = 0, Cb = 0, TC = 0, nstep = 0, NBr = 0, NBc = 0, nred = 0, sqchunk = 0;
     int32_t cb_hint = 0;
+    int32_t interleave = 1;
     bool uploaded = false, started = false;
     bool use_graph = false;
     std::map<int64_t, hipGraphExec_t> graphs;
@@ -782,8 +852,8 @@ int setup_geometry(pdot_ctx* ctx) {
     ctx->Cb = (int32_t)Cb;
     ctx->TC = (int32_t)TC;
     ctx->nstep = (int32_t)(R * Cb);
-    ctx->NBr = (int32_t)((ctx->n + kRedBlock - 1) / kRedBlock);
-    ctx->NBc = (int32_t)((ctx->m + kRedBlock - 1) / kRedBlock);
+    ctx->NBr = (int32_t)((ctx->n + kWave - 1) / kWave);
+    ctx->NBc = (int32_t)((ctx->m + kWave - 1) / kWave);
     ctx->nred = ctx->NBr + ctx->NBc;
     ctx->sqchunk = (ctx->nstep + ctx->nred - 1) / ctx->nred;
     int rc;
@@ -811,6 +881,7 @@ StepParams step_params(pdot_ctx* ctx) {
     p.TC = ctx->TC;
     p.Cb = ctx->Cb;
     p.R = ctx->R;
+    p.interleave = ctx->interleave;
     return p;
 }
 
@@ -861,7 +932,7 @@ hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
 }
 
 hipError_t launch_commit(pdot_ctx* ctx) {
-    hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(kRedBlock), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
+    hipLaunchKernelGGL(commit_kernel, dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
     return hipGetLastError();
 }
 
@@ -1244,9 +1315,10 @@ int pdot_use_graphs(pdot_ctx* ctx, int32_t enable) {
     return PDOT_OK;
 }
 
-int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint) {
+int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint, int32_t interleave) {
     if (!ctx) return PDOT_E_ARG;
     if (col_blocks_hint < 0) return fail(ctx, PDOT_E_ARG, "pdot_set_launch: negative hint");
+    if (interleave != 0 && interleave != 1) return fail(ctx, PDOT_E_ARG, "pdot_set_launch: interleave must be 0 or 1");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     // the pending pass (if any) keeps its partials only while the geometry is unchanged
@@ -1257,6 +1329,7 @@ int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint) {
         if (st.pending) return fail(ctx, PDOT_E_STATE, "pdot_set_launch: a pass is pending");
     }
     ctx->cb_hint = col_blocks_hint;
+    ctx->interleave = interleave;
     return setup_geometry(ctx);
 }
 
@@ -1270,5 +1343,40 @@ int pdot_get_launch(pdot_ctx* ctx, int32_t* rows_per_blk, int32_t* col_blocks, i
 }
 
 int64_t pdot_device_bytes(const pdot_ctx* ctx) { return ctx ? ctx->bytes : 0; }
+
+int pdot_copy_probe(pdot_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy) {
+    if (!ctx) return PDOT_E_ARG;
+    if (bytes <= 0 || bytes % 16 != 0 || reps < 1) return fail(ctx, PDOT_E_ARG, "pdot_copy_probe: bytes must be a positive multiple of 16");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    void *src = nullptr, *dst = nullptr;
+    int rc;
+    if ((rc = dev_alloc(ctx, &src, (size_t)bytes))) return rc;
+    if ((rc = dev_alloc(ctx, &dst, (size_t)bytes))) {
+        hipFree(src);
+        return rc;
+    }
+    HIPCHK(ctx, hipMemsetAsync(src, 0, (size_t)bytes, ctx->stream));
+    const int64_t nv = bytes / 16;
+    const unsigned grid = (unsigned)((nv + 255) / 256);
+    hipEvent_t e0, e1;
+    HIPCHK(ctx, hipEventCreate(&e0));
+    HIPCHK(ctx, hipEventCreate(&e1));
+    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const dbl2*)src, (dbl2*)dst, nv);
+    HIPCHK(ctx, hipEventRecord(e0, ctx->stream));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(256), 0, ctx->stream, (const dbl2*)src, (dbl2*)dst, nv);
+    HIPCHK(ctx, hipGetLastError());
+    HIPCHK(ctx, hipEventRecord(e1, ctx->stream));
+    HIPCHK(ctx, hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, e0, e1));
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(src);
+    hipFree(dst);
+    ctx->bytes -= 2 * bytes;
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    return PDOT_OK;
+}
 
 }  // extern "C"

@@ -102,9 +102,10 @@ SIGNATURES = {
     "pdot_download": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p]),
     "pdot_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
     "pdot_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
-    "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
+    "pdot_copy_probe": (ctypes.c_int, [_ctx_p, ctypes.c_int64, ctypes.c_int32, _c_double_p]),
 }
 
 _lib = None
@@ -232,8 +233,8 @@ class DeviceSolver:
     def use_graphs(self, enable: bool) -> None:
         self._check(self.lib.pdot_use_graphs(self.h, int(bool(enable))), "pdot_use_graphs")
 
-    def set_launch(self, col_blocks_hint: int) -> None:
-        self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint)), "pdot_set_launch")
+    def set_launch(self, col_blocks_hint: int = 0, interleave: bool = True) -> None:
+        self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint), int(bool(interleave))), "pdot_set_launch")
 
     def get_launch(self) -> dict:
         vals = [ctypes.c_int32() for _ in range(4)]
@@ -242,3 +243,9 @@ class DeviceSolver:
 
     def device_bytes(self) -> int:
         return int(self.lib.pdot_device_bytes(self.h))
+
+    def copy_probe(self, nbytes: int, reps: int = 10) -> float:
+        """Mean milliseconds of a plain device copy of nbytes (2*nbytes of traffic)."""
+        ms = ctypes.c_double()
+        self._check(self.lib.pdot_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(ms)), "pdot_copy_probe")
+        return ms.value

