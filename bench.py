@@ -41,6 +41,19 @@ def algorithmic_bytes_per_step(n: int, m: int, elem: int) -> int:
     return 5 * n * m * elem + 10 * 8 * (n + m)
 
 
+def measured_traffic(n: int, m: int, form: str):
+    """HBM bytes per step-kernel launch measured with rocprofv3 FETCH_SIZE /
+    WRITE_SIZE (scaled by a copy kernel of known size) for this workload, as
+    recorded in profiles/hbm_traffic_step_kernel_gaussian_shift_10k.md; None
+    for other sizes."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_step_kernel_gaussian_shift_10k.md")
+    if (n, m) != (10000, 10000) or not os.path.exists(path):
+        return None
+    text = open(path).read()
+    block = text.split("```json", 1)[1].split("```", 1)[0]
+    return json.loads(block)[form]["traffic_bytes"]
+
+
 def measure_steps(dev, steps: int, warmup: int):
     from pdot_amd import _lib
     dev.set_start(policy=_lib.POLICY_ADAPTIVE, step_size=1.0, primal_weight=1.0)
@@ -156,7 +169,7 @@ def main() -> int:
         "config": {"workload": f"gaussian_shift {n}x{m} FP64 (squared Euclidean / max, U(0,1) masses, seed 2000)",
                    "n": n, "m": m, "storage": "f64", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": f64["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": f64["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": None,
+                     "frac": f64["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n, m, "f64"),
                      "kernel": "step_kernel<double>", "bytes_per_launch": f64["bytes_per_step"],
                      "avg_launch_us": f64["step_kernel_us"]},
         "cpu_baseline": cpu,
