@@ -45,9 +45,15 @@
 namespace {
 
 constexpr int kWave = 64;
-constexpr int kWaves = 4;
+#ifndef PDOT_STEP_WAVES
+#define PDOT_STEP_WAVES 8
+#endif
+constexpr int kWaves = PDOT_STEP_WAVES;  // step kernel: waves per block, stacked along rows
 constexpr int kBlock = kWave * kWaves;
-constexpr int kRedWaves = 4;                 // reduce kernel: waves per block
+#ifndef PDOT_RED_WAVES
+#define PDOT_RED_WAVES 8
+#endif
+constexpr int kRedWaves = PDOT_RED_WAVES;     // reduce kernel: waves per block
 constexpr int kRedBlock = kWave * kRedWaves;  // each block owns 64 rows or columns
 constexpr int kChunk = 64;        // columns per LDS flush of column sums
 constexpr int kLdAlign = 64;      // leading-dimension alignment (elements)
@@ -55,7 +61,7 @@ constexpr int kMatStats = 10;     // per-block matrix eval reductions
 constexpr int kVecStats = 12;     // per-block vector eval reductions
 constexpr int kEvalOut = kMatStats + kVecStats;
 #ifndef PDOT_STEP_UNROLL
-#define PDOT_STEP_UNROLL 1
+#define PDOT_STEP_UNROLL 2
 #endif
 constexpr int kUnroll = PDOT_STEP_UNROLL;  // columns whose loads are issued together
 
@@ -509,7 +515,9 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
         }
         __syncthreads();
         if (do_step && tid < tend - tc) {
-            const double t = ((colbuf[0][tid] + colbuf[1][tid]) + colbuf[2][tid]) + colbuf[3][tid];
+            double t = colbuf[0][tid];
+#pragma unroll
+            for (int ww = 1; ww < kWaves; ++ww) t += colbuf[ww][tid];
             p.colpart[(int64_t)rb * m + j0 + (tc + tid) * jstep] = t;
         }
         __syncthreads();
@@ -568,20 +576,21 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     // (one per wave), each summed with four independent accumulators so that
     // several loads are in flight; combined below in a fixed order.
     if (do_step) {
-        double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+        constexpr int A = 8;  // independent accumulators: A loads in flight per lane
+        double acc[A];
+#pragma unroll
+        for (int q = 0; q < A; ++q) acc[q] = 0.0;
         if (k < len) {
             const int t0 = (int)(((int64_t)np * w) / kRedWaves);
             const int t1 = (int)(((int64_t)np * (w + 1)) / kRedWaves);
             int t = t0;
-            for (; t + 4 <= t1; t += 4) {
-                a0 += part[(int64_t)t * pstride + k];
-                a1 += part[(int64_t)(t + 1) * pstride + k];
-                a2 += part[(int64_t)(t + 2) * pstride + k];
-                a3 += part[(int64_t)(t + 3) * pstride + k];
+            for (; t + A <= t1; t += A) {
+#pragma unroll
+                for (int q = 0; q < A; ++q) acc[q] += part[(int64_t)(t + q) * pstride + k];
             }
-            for (; t < t1; ++t) a0 += part[(int64_t)t * pstride + k];
+            for (; t < t1; ++t) acc[0] += part[(int64_t)t * pstride + k];
         }
-        psum[w][lane] = (a0 + a1) + (a2 + a3);
+        psum[w][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
     }
     __syncthreads();
 
@@ -804,6 +813,7 @@ int fail(pdot_ctx* ctx, int code, const std::string& msg) {
     } while (0)
 
 int rows_per_block(int32_t storage) { return kWaves * kWave * (storage == PDOT_STORAGE_F64 ? 2 : 4); }
+static_assert(kWaves >= 1 && kWaves <= 16, "step kernel waves");
 
 int dev_alloc(pdot_ctx* ctx, void** p, size_t bytes) {
     if (bytes == 0) bytes = 8;
