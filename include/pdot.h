@@ -67,6 +67,8 @@ typedef struct pdot_solver_state {
     int32_t avg_slot;             /* internal: slot holding the average (valid if avg_count > 0) */
     int32_t pending;              /* internal: a speculative pass awaits commit */
     int32_t spare0, spare1;       /* internal: slots written by the pending pass */
+    int64_t step_budget;          /* internal (batches): passes left in the current run */
+    int64_t reserved;
 } pdot_solver_state;
 
 /* Reductions of one point (X, u, v, r, c, G) with G = (C - u 1^T) - 1 v^T,
@@ -170,6 +172,55 @@ int64_t pdot_device_bytes(const pdot_ctx* ctx);
  * Used to calibrate HBM byte counters and bandwidth against a kernel of known
  * traffic (2 * bytes per copy). */
 int pdot_copy_probe(pdot_ctx* ctx, int64_t bytes, int32_t reps, double* ms_per_copy);
+
+/* ------------------------------------------------------------------------
+ * Batches: many independent problems of uneven n x m advanced together.
+ * No counterpart in the reference, which solves one problem per optimize
+ * call (src/solver_gpu.jl:413); every instance follows exactly the
+ * single-problem semantics above, with its own step size, primal weight,
+ * averaging weights, restart and termination state.  Per-instance arrays
+ * have `count` entries.
+ * ------------------------------------------------------------------------ */
+typedef struct pdot_batch pdot_batch;
+
+int pdot_batch_create(pdot_batch** out, int32_t count, const int64_t* n, const int64_t* m, int32_t storage,
+                      int32_t device);
+void pdot_batch_destroy(pdot_batch* b);
+const char* pdot_batch_last_error(const pdot_batch* b);
+/* ot_cpu_to_gpu for instance k (src/io.jl:18-36). */
+int pdot_batch_upload(pdot_batch* b, int32_t k, const double* C, const double* a, const double* bvec);
+/* Zero start for every instance (src/solver_gpu.jl:428-496); step_size and
+ * primal_weight are per instance.  All instances become active. */
+int pdot_batch_set_start(pdot_batch* b, int32_t step_policy, double reduction_exponent, double growth_exponent,
+                         const double* step_size, const double* primal_weight);
+/* Optional warm start (u0, v0) of instance k, after pdot_batch_set_start and
+ * before any pass: X0 = 0, G0 = (C - u0) - v0. */
+int pdot_batch_set_warm_start(pdot_batch* b, int32_t k, const double* u0, const double* v0);
+/* Active instances take part in passes and evaluations; inactive ones (for
+ * example terminated) are skipped.  Rebuilds the work lists. */
+int pdot_batch_set_active(pdot_batch* b, const int32_t* active);
+/* counts[k] take_step! calls for each active instance (the pending pass of an
+ * evaluation is committed first); instances run concurrently in one launch
+ * sequence of max(counts) passes. */
+int pdot_batch_run_steps(pdot_batch* b, const int64_t* counts, pdot_solver_state* out);
+/* pdot_eval for every active instance. */
+int pdot_batch_eval(pdot_batch* b, pdot_point_stats* current, pdot_point_stats* average, pdot_solver_state* states);
+/* Per instance: action 0 none, 1 restart to the current point, 2 restart to
+ * the average (src/saddle_point_gpu.jl:639-696); new_primal_weight applies
+ * where action != 0. */
+int pdot_batch_restart(pdot_batch* b, const int32_t* action, const double* new_primal_weight);
+int pdot_batch_discard(pdot_batch* b, const int32_t* which);
+int pdot_batch_get_states(pdot_batch* b, pdot_solver_state* out);
+int pdot_batch_set_states(pdot_batch* b, const pdot_solver_state* in);
+int pdot_batch_download(pdot_batch* b, int32_t k, int32_t which, double* X, double* u, double* v);
+int pdot_batch_time_steps(pdot_batch* b, int64_t count, double* step_kernel_ms, double* total_ms);
+int pdot_batch_use_graphs(pdot_batch* b, int32_t enable);
+/* Launch shape of the persistent step kernel (tuned by pdot_amd.batch.autotune):
+ * columns per tile and the grid as a multiple of the resident block count. */
+int pdot_batch_set_launch(pdot_batch* b, int32_t cols_per_tile, int32_t grid_mult);
+int pdot_batch_get_launch(pdot_batch* b, int32_t* rows_per_tile, int32_t* cols_per_tile, int32_t* tiles,
+                          int32_t* step_grid);
+int64_t pdot_batch_device_bytes(const pdot_batch* b);
 
 #ifdef __cplusplus
 }

@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRC = os.path.join(HERE, "csrc", "pdot_hip.hip")
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("pdot_hip.hip", "pdot_batch.hip")]
 OUT = os.path.join(HERE, "pdot_amd", "libpdot_hip.so")
 
 HIPCC_FLAGS = [
@@ -30,7 +30,7 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(REPO, "include"), "-o", OUT + ".tmp", SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(REPO, "include"), "-o", OUT + ".tmp", *SRCS]
     if verbose:
         print(" ".join(cmd))
     res = subprocess.run(cmd, capture_output=True, text=True)

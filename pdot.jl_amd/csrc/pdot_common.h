@@ -1,0 +1,604 @@
+// pdot_common.h -- device code shared by the single-instance and the batched
+// PDHG kernels of libpdot_hip.so (gfx950): Julia-semantics scalar helpers,
+// fixed-order DPP/LDS reductions, the step-size controller, and the bodies of
+// one step tile and one reduce chunk.  Included by pdot_hip.hip and
+// pdot_batch.hip; everything lives in an anonymous namespace, one copy per
+// translation unit.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "pdot.h"
+
+namespace {
+
+constexpr int kWave = 64;
+#ifndef PDOT_STEP_WAVES
+#define PDOT_STEP_WAVES 8
+#endif
+constexpr int kWaves = PDOT_STEP_WAVES;  // step kernel: waves per block, stacked along rows
+constexpr int kBlock = kWave * kWaves;
+#ifndef PDOT_RED_WAVES
+#define PDOT_RED_WAVES 8
+#endif
+constexpr int kRedWaves = PDOT_RED_WAVES;     // reduce kernel: waves per block
+constexpr int kRedBlock = kWave * kRedWaves;  // each block owns 64 rows or columns
+constexpr int kChunk = 64;        // columns per LDS flush of column sums
+constexpr int kLdAlign = 64;      // leading-dimension alignment (elements)
+constexpr int kMatStats = 10;     // per-block matrix eval reductions
+constexpr int kVecStats = 12;     // per-block vector eval reductions
+constexpr int kEvalOut = kMatStats + kVecStats;
+#ifndef PDOT_STEP_UNROLL
+#define PDOT_STEP_UNROLL 2
+#endif
+constexpr int kUnroll = PDOT_STEP_UNROLL;  // columns whose loads are issued together
+
+// ------------------------------------------------------------------ helpers
+
+// Julia's max(x, 0.0) / min(x, 0.0) / min(x, y) propagate NaN.
+__device__ __forceinline__ double jl_max0(double x) { return x != x ? x : (x > 0.0 ? x : 0.0); }
+__device__ __forceinline__ double jl_min0(double x) { return x != x ? x : (x < 0.0 ? x : 0.0); }
+__device__ __forceinline__ double jl_min(double x, double y) {
+    if (x != x || y != y) return __longlong_as_double(0x7ff8000000000000LL);
+    return x < y ? x : y;
+}
+// max of non-negative values that keeps a NaN once seen
+__device__ __forceinline__ double nanmax(double acc, double t) { return (acc != acc || t != t || t > acc) ? (acc != acc ? acc : t) : acc; }
+
+// (k+1)^e as Julia's Int^Float64: integral exponents by repeated products.
+__device__ inline double pow_int_base(double x, double e) {
+    if (e == floor(e) && fabs(e) <= 64.0) {
+        int k = (int)fabs(e);
+        double r = 1.0;
+        for (int t = 0; t < k; ++t) r *= x;
+        return e < 0 ? 1.0 / r : r;
+    }
+    return pow(x, e);
+}
+
+// DPP lane movement of a double (two 32-bit halves); lanes outside row_mask
+// or reading an invalid source receive 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+    int lo = __double2loint(x);
+    int hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xF, false);
+    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double x, int lane) {
+    int lo = __builtin_amdgcn_readlane(__double2loint(x), lane);
+    int hi = __builtin_amdgcn_readlane(__double2hiint(x), lane);
+    return __hiloint2double(hi, lo);
+}
+
+// Sum over the 64 lanes of a wave in a fixed DPP tree; every lane gets the
+// total.  Must be called with all lanes active.
+__device__ __forceinline__ double wave_sum(double x) {
+    x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+    x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+    x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+    x += dpp_f64<0x140, 0xF>(x);  // row_mirror
+    x += dpp_f64<0x142, 0xA>(x);  // row_bcast:15
+    x += dpp_f64<0x143, 0xC>(x);  // row_bcast:31
+    return readlane_f64(x, 63);
+}
+__device__ __forceinline__ double wave_max(double x) {
+    x = nanmax(x, dpp_f64<0xB1, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x4E, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x141, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x140, 0xF>(x));
+    x = nanmax(x, dpp_f64<0x142, 0xA>(x));
+    x = nanmax(x, dpp_f64<0x143, 0xC>(x));
+    return readlane_f64(x, 63);
+}
+
+// Deterministic block-wide sum/max (power-of-two block size); result valid in all threads.
+__device__ inline double block_sum(double v, double* red) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = v;
+    __syncthreads();
+    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = red[tid] + red[tid + s];
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+__device__ inline double block_max(double v, double* red) {
+    const int tid = threadIdx.x;
+    __syncthreads();
+    red[tid] = v;
+    __syncthreads();
+    for (int s = (int)blockDim.x / 2; s > 0; s >>= 1) {
+        if (tid < s) red[tid] = nanmax(red[tid], red[tid + s]);
+        __syncthreads();
+    }
+    double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// --------------------------------------------------------------- controller
+
+// The two slots a pass writes: the first slots holding neither the current
+// point nor the (non-empty) average.
+__device__ __forceinline__ void choose_spares(pdot_solver_state& st) {
+    int used0 = st.cur_slot, used1 = st.avg_count > 0 ? st.avg_slot : st.cur_slot;
+    int k = 0;
+    for (int s = 0; s < 4; ++s) {
+        if (s == used0 || s == used1) continue;
+        if (k == 0) st.spare0 = s;
+        else if (k == 1) st.spare1 = s;
+        ++k;
+    }
+}
+
+// Commit the pending pass: take_step!'s acceptance test and step-size
+// recurrence (src/solver_gpu.jl:296-366), its replay of a rejected pass (the
+// replay recomputes an identical pass, so only the scalars change), and the
+// swap of the spare slots into the current point and the average
+// (update_solution_in_solver_state!, :237-264, whose averaging weight is the
+// pre-step eta).  A pass with interaction > 0 and movement == 0 sets
+// numerical_error and is not applied (:338-344); a NaN step-size limit, on
+// which the reference loops forever, is reported the same way.
+__device__ inline void controller_commit(pdot_solver_state& st, double dot, double yy, double dx2) {
+    st.pending = 0;
+    if (st.step_budget > 0) st.step_budget -= 1;
+    const double eta = st.step_size;
+    const double omega = st.primal_weight;
+    bool commit;
+    st.take_step_calls += 1;
+    st.last_dx2 = dx2;
+    st.last_dot = dot;
+    st.last_dy2 = yy;
+    if (st.step_policy == PDOT_POLICY_CONSTANT) {
+        st.cumulative_kkt_passes += 1;
+        commit = true;
+    } else {
+        const double interaction = fabs(dot);
+        const double nd = sqrt(dx2);
+        const double ny = sqrt(yy);
+        const double movement = 0.5 * omega * (nd * nd) + (0.5 / omega) * (ny * ny);
+        st.last_interaction = interaction;
+        st.last_movement = movement;
+        double L = __longlong_as_double(0x7ff0000000000000LL);
+        double eta_loc = eta;
+        bool done = false, replay = false, ne = false;
+        for (int pass = 0; pass < 4 && !done; ++pass) {
+            st.total_number_iterations += 1;
+            st.cumulative_kkt_passes += 1;
+            if (!replay) {
+                if (interaction > 0) {
+                    L = movement / interaction;
+                    if (movement == 0.0) {
+                        ne = true;
+                        break;
+                    }
+                } else {
+                    L = __longlong_as_double(0x7ff0000000000000LL);
+                }
+            }
+            if (eta_loc <= L) done = true;
+            const double k1 = (double)(st.total_number_iterations + 1);
+            const double first = (1 - 1 / pow_int_base(k1, st.reduction_exponent)) * L;
+            const double second = (1 + 1 / pow_int_base(k1, st.growth_exponent)) * eta_loc;
+            eta_loc = jl_min(first, second);
+            replay = true;
+            if (!done && eta_loc != eta_loc) {
+                ne = true;
+                break;
+            }
+        }
+        if (!done) ne = true;
+        st.last_step_limit = L;
+        st.step_size = eta_loc;
+        if (ne) {
+            st.numerical_error = 1;
+            st.error_call = st.take_step_calls;
+        }
+        commit = done;
+    }
+    if (commit) {
+        st.avg_slot = (st.avg_count == 0) ? st.spare0 : st.spare1;
+        st.cur_slot = st.spare0;
+        st.sum_weights += eta;
+        st.avg_count += 1;
+        st.gzero = 0;
+        choose_spares(st);
+    }
+}
+
+// Reduce the per-block scalars of the reduce kernel (dot, ||dy||^2, ||dX||^2
+// chunk) in a fixed order and commit.  Called by the 64 lanes of one wave;
+// lane 0 updates *sst.
+__device__ inline void reduce_and_commit(pdot_solver_state* sst, const double* redpart, int nred) {
+    const int tid = threadIdx.x;
+    double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+    for (int k = tid; k < nred; k += kWave) {
+        d0 += redpart[3 * k + 0];
+        d1 += redpart[3 * k + 1];
+        d2 += redpart[3 * k + 2];
+    }
+    d0 = wave_sum(d0);
+    d1 = wave_sum(d1);
+    d2 = wave_sum(d2);
+    if (tid == 0) controller_commit(*sst, d0, d1, d2);
+}
+
+// ---------------------------------------------------------------- vectors
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+typedef float flt4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct VecT;
+template <> struct VecT<double> {
+    static constexpr int N = 2;
+    using type = dbl2;
+};
+template <> struct VecT<float> {
+    static constexpr int N = 4;
+    using type = flt4;
+};
+
+#ifndef PDOT_NT
+#define PDOT_NT 1
+#endif
+// 16-byte loads/stores of the streamed matrices; with PDOT_NT the accesses
+// carry the nontemporal hint (the matrices are touched once per pass).
+template <typename V>
+__device__ __forceinline__ V ld16(const V* p) {
+    if constexpr (PDOT_NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+template <typename V>
+__device__ __forceinline__ void st16(V v, V* p) {
+    if constexpr (PDOT_NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+
+template <typename V, typename T>
+__device__ __forceinline__ void unpack(const V& v, double* out);
+template <>
+__device__ __forceinline__ void unpack<dbl2, double>(const dbl2& v, double* o) {
+    o[0] = v.x;
+    o[1] = v.y;
+}
+template <>
+__device__ __forceinline__ void unpack<flt4, float>(const flt4& v, double* o) {
+    o[0] = (double)v.x;
+    o[1] = (double)v.y;
+    o[2] = (double)v.z;
+    o[3] = (double)v.w;
+}
+
+template <typename T>
+__device__ __forceinline__ void store_vec(T* p, const double* o);
+template <>
+__device__ __forceinline__ void store_vec<double>(double* p, const double* o) {
+    dbl2 v = {o[0], o[1]};
+    st16(v, reinterpret_cast<dbl2*>(p));
+}
+template <>
+__device__ __forceinline__ void store_vec<float>(float* p, const double* o) {
+    flt4 v = {(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
+    st16(v, reinterpret_cast<flt4*>(p));
+}
+
+// ------------------------------------------------------------ step tile
+
+// Everything one block needs to advance one tile of one problem by a PDHG
+// pass: rows [row0, row0 + WAVES*64*VEC) of the columns j0 + t*jstep,
+// t < ncols.
+template <typename T>
+struct TileArgs {
+    const T* C;
+    const T* Xc;   // current point
+    const T* Xa;   // running average (read when !avg_empty or EVAL)
+    T* Xn;         // new point
+    T* Xan;        // new average
+    const double* uc;
+    const double* ua;
+    const double* vc;
+    const double* va;
+    double* rowpart;  // partial row sums of this tile, indexed by row
+    double* colpart;  // partial column sums of this tile, indexed by column
+    int64_t n, ld, row0, j0, jstep, ncols;
+    double eta, omega, nw, aw;
+    bool avg_empty, gzero, do_step;
+};
+
+// One PDHG pass over a tile (src/solver_gpu.jl:125-174 and 237-264,
+// src/saddle_point_gpu.jl:115-135): each lane owns VEC consecutive rows and
+// walks the tile's columns.  Row sums accumulate in registers, column sums
+// are reduced per column across the wave (DPP) and across the block's waves
+// (LDS, every kChunk columns).  With EVAL the same loads also produce the
+// evaluation reductions of the input point and of its average.  Called by
+// all threads of the block; thread 0 receives ||Delta||^2 of the tile in
+// *dd_out and the kMatStats evaluation reductions in ev_out.
+template <typename T, bool EVAL, int UNROLL, int WAVES>
+__device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)[kChunk], double* red, double* dd_out,
+                                          double* ev_out) {
+    constexpr int VEC = VecT<T>::N;
+    using V = typename VecT<T>::type;
+    constexpr int WROWS = kWave * VEC;
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    const int64_t n = a.n, ld = a.ld;
+    const int64_t i0 = a.row0 + (int64_t)w * WROWS + (int64_t)lane * VEC;
+    const bool lane_active = i0 < n;
+    const bool avg_empty = a.avg_empty;
+    const bool do_step = a.do_step;
+    const T* __restrict__ C = a.C;
+    const T* __restrict__ Xc = a.Xc;
+    const T* __restrict__ Xa = a.Xa;
+    T* __restrict__ Xn = a.Xn;
+    T* __restrict__ Xan = a.Xan;
+    const double* __restrict__ vc = a.vc;
+    const double* __restrict__ va = a.va;
+    const double eta = a.eta, omega = a.omega, nw = a.nw, aw = a.aw;
+    const bool gzero = a.gzero;
+    const int64_t j0 = a.j0, jstep = a.jstep, ncols = a.ncols;
+
+    double ui[VEC], uai[VEC];
+    bool valid[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) {
+        valid[e] = (i0 + e) < n;
+        ui[e] = (lane_active && valid[e]) ? a.uc[i0 + e] : 0.0;
+        uai[e] = (EVAL && lane_active && valid[e]) ? a.ua[i0 + e] : 0.0;
+    }
+    double rs[VEC];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) rs[e] = 0.0;
+    double dd = 0.0;
+    double ev[kMatStats];
+#pragma unroll
+    for (int k = 0; k < kMatStats; ++k) ev[k] = 0.0;
+
+    for (int64_t tc = 0; tc < ncols; tc += kChunk) {
+        const int64_t tend = (tc + kChunk < ncols) ? tc + kChunk : ncols;
+        for (int64_t tb = tc; tb < tend; tb += UNROLL) {
+            V xv[UNROLL], cv[UNROLL], av[UNROLL];
+            double vj[UNROLL], vaj[UNROLL];
+#pragma unroll
+            for (int q = 0; q < UNROLL; ++q) {
+                const bool colok = tb + q < tend;
+                const int64_t j = j0 + (tb + q) * jstep;
+                vj[q] = colok ? vc[j] : 0.0;
+                vaj[q] = (EVAL && colok) ? va[j] : 0.0;
+                if (colok && lane_active) {
+                    const int64_t idx = j * ld + i0;
+                    xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
+                    cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
+                    if (!avg_empty) av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < UNROLL; ++q) {
+                const bool colok = tb + q < tend;
+                const int64_t j = j0 + (tb + q) * jstep;
+                double csl = 0.0;
+                if (colok && lane_active) {
+                    const int64_t idx = j * ld + i0;
+                    double x[VEC], c[VEC], xa[VEC], xn[VEC], xan[VEC];
+                    unpack<V, T>(xv[q], x);
+                    unpack<V, T>(cv[q], c);
+                    if (!avg_empty) unpack<V, T>(av[q], xa);
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) {
+                        // G_ij = (C_ij - u_i) - v_j (solver_gpu.jl:249-250); 0 at start (:434)
+                        double g = (gzero || !valid[e]) ? 0.0 : (c[e] - ui[e]) - vj[q];
+                        if (EVAL) {
+                            ev[0] += c[e] * x[e];
+                            ev[1] = nanmax(ev[1], fabs(x[e]));
+                            ev[2] += x[e] * x[e];
+                            const double rc = jl_min0(g);
+                            ev[3] = nanmax(ev[3], fabs(rc));
+                            ev[4] += rc * rc;
+                            if (!avg_empty) {
+                                const double ga = valid[e] ? (c[e] - uai[e]) - vaj[q] : 0.0;
+                                ev[5] += c[e] * xa[e];
+                                ev[6] = nanmax(ev[6], fabs(xa[e]));
+                                ev[7] += xa[e] * xa[e];
+                                const double rca = jl_min0(ga);
+                                ev[8] = nanmax(ev[8], fabs(rca));
+                                ev[9] += rca * rca;
+                            }
+                        }
+                        // Delta = max(X - (eta/omega) G, 0) - X  (solver_gpu.jl:139-141)
+                        double d = x[e] - (eta / omega) * g;
+                        d = jl_max0(d);
+                        double xnew;
+                        if constexpr (sizeof(T) == 8) {
+                            d -= x[e];
+                            xnew = x[e] + d;  // X .+= Delta (solver_gpu.jl:242)
+                        } else {
+                            xnew = (double)(float)d;  // FP32 storage: Delta of the stored values
+                            d = xnew - x[e];
+                        }
+                        xn[e] = xnew;
+                        rs[e] += d;
+                        csl += d;
+                        dd += d * d;
+                        // avg = (avg + x*nw)*aw (saddle_point_gpu.jl:129-130)
+                        if (!avg_empty) {
+                            double t = xa[e] + xnew * nw;
+                            xan[e] = t * aw;
+                        }
+                    }
+                    if (do_step) {
+                        store_vec<T>(Xn + idx, xn);
+                        if (!avg_empty) store_vec<T>(Xan + idx, xan);
+                    }
+                }
+                const double s = wave_sum(csl);
+                if (colok && lane == 0) colbuf[w][tb + q - tc] = s;
+            }
+        }
+        __syncthreads();
+        if (do_step && tid < tend - tc) {
+            double t = colbuf[0][tid];
+#pragma unroll
+            for (int ww = 1; ww < WAVES; ++ww) t += colbuf[ww][tid];
+            a.colpart[j0 + (tc + tid) * jstep] = t;
+        }
+        __syncthreads();
+    }
+
+    if (do_step && lane_active) {
+#pragma unroll
+        for (int e = 0; e < VEC; ++e) a.rowpart[i0 + e] = rs[e];
+    }
+    const double ddb = do_step ? block_sum(dd, red) : 0.0;
+    if (tid == 0) *dd_out = ddb;
+    if (EVAL) {
+#pragma unroll
+        for (int k = 0; k < kMatStats; ++k) {
+            const bool is_max = (k == 1 || k == 3 || k == 6 || k == 8);
+            const double t = is_max ? block_max(ev[k], red) : block_sum(ev[k], red);
+            if (tid == 0) ev_out[k] = t;
+        }
+    }
+}
+
+// --------------------------------------------------------- reduce chunk
+
+// The rows (or columns) [k0, k0 + 64) of one problem: their partial sums,
+// the slot vectors they update, and this chunk's share of the per-tile
+// ||Delta||^2 values.
+struct ChunkArgs {
+    const double* part;  // partials: part[t * pstride + k], t < np
+    int64_t pstride;
+    int32_t np;
+    int64_t k0, len;
+    bool rows;           // rows (u, r, a) or columns (v, c, b)
+    const double* q;     // a or b
+    const double* y_cur;
+    const double* s_cur;
+    const double* y_avg;
+    const double* s_avg;
+    double* y_new;
+    double* s_new;
+    double* y_anew;
+    double* s_anew;
+    const double* sqpart;
+    int64_t sq_lo, sq_hi;
+    double eta, omega, nw, aw;
+    bool avg_empty, do_step;
+};
+
+// Finishes one pass for a chunk (src/solver_gpu.jl:172-173, 177-234,
+// 267-288; src/saddle_point_gpu.jl:140-221): r+ = rowsum(Delta) + r,
+// u+ = u + eta*omega*(a - 2 r+ + r), the averages of u, v, r, c, and the
+// chunk's parts of dr'du + dc'dv, ||dy||^2 and ||Delta||^2.  The partials are
+// split into RWAVES contiguous ranges (one per wave) summed with eight
+// independent accumulators, then combined in a fixed order by wave 0.  With
+// EVAL, also the vector parts of the evaluation of the input point and its
+// average.  Called by all threads; thread 0 receives out3 = {dot, yy, dx}
+// and ev_out[kVecStats].
+template <bool EVAL, int RWAVES>
+__device__ __forceinline__ void reduce_chunk(const ChunkArgs& c, double (*psum)[kWave], double* out3, double* ev_out) {
+    const int tid = threadIdx.x;
+    const int lane = tid & (kWave - 1);
+    const int w = tid / kWave;
+    const int64_t k = c.k0 + lane;
+    const bool do_step = c.do_step;
+    if (do_step) {
+        constexpr int A = 8;  // independent accumulators: A loads in flight per lane
+        double acc[A];
+#pragma unroll
+        for (int q = 0; q < A; ++q) acc[q] = 0.0;
+        if (k < c.len) {
+            const int t0 = (int)(((int64_t)c.np * w) / RWAVES);
+            const int t1 = (int)(((int64_t)c.np * (w + 1)) / RWAVES);
+            int t = t0;
+            for (; t + A <= t1; t += A) {
+#pragma unroll
+                for (int q = 0; q < A; ++q) acc[q] += c.part[(int64_t)(t + q) * c.pstride + k];
+            }
+            for (; t < t1; ++t) acc[0] += c.part[(int64_t)t * c.pstride + k];
+        }
+        psum[w][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    }
+    __syncthreads();
+    if (w == 0) {
+        const bool avg_empty = c.avg_empty;
+        const double eta = c.eta, omega = c.omega, nw = c.nw, aw = c.aw;
+        double dot = 0.0, yy = 0.0;
+        double ev[kVecStats];
+#pragma unroll
+        for (int t = 0; t < kVecStats; ++t) ev[t] = 0.0;
+        if (k < c.len) {
+            const bool rows = c.rows;
+            const double q = c.q[k];
+            const double s_k = c.s_cur[k];
+            const double y_k = c.y_cur[k];
+            if (do_step) {
+                double sum = psum[0][lane];
+#pragma unroll
+                for (int t = 1; t < RWAVES; ++t) sum += psum[t][lane];
+                const double sn = sum + s_k;
+                const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
+                const double ds = sn - s_k, dy = yn - y_k;
+                dot = ds * dy;
+                yy = dy * dy;
+                c.s_new[k] = sn;
+                c.y_new[k] = yn;
+                if (!avg_empty) {
+                    const double t1 = c.s_avg[k] + sn * nw;
+                    c.s_anew[k] = t1 * aw;
+                    const double t2 = c.y_avg[k] + yn * nw;
+                    c.y_anew[k] = t2 * aw;
+                }
+            }
+            if (EVAL) {
+                const double t = q - s_k;
+                ev[0] = fabs(t);
+                ev[1] = t * t;
+                ev[2] = q * y_k;
+                ev[3] = rows ? fabs(y_k) : 0.0;
+                ev[4] = rows ? y_k * y_k : 0.0;
+                ev[5] = y_k * y_k;
+                if (!avg_empty) {
+                    const double sa = c.s_avg[k], ya = c.y_avg[k];
+                    const double ta = q - sa;
+                    ev[6] = fabs(ta);
+                    ev[7] = ta * ta;
+                    ev[8] = q * ya;
+                    ev[9] = rows ? fabs(ya) : 0.0;
+                    ev[10] = rows ? ya * ya : 0.0;
+                    ev[11] = ya * ya;
+                }
+            }
+        }
+        if (EVAL) {
+#pragma unroll
+            for (int t = 0; t < kVecStats; ++t) {
+                const bool is_max = (t == 0 || t == 3 || t == 6 || t == 9);
+                const double v = is_max ? wave_max(ev[t]) : wave_sum(ev[t]);
+                if (lane == 0) ev_out[t] = v;
+            }
+        }
+        if (do_step) {
+            double dx = 0.0;
+            for (int64_t t = c.sq_lo + lane; t < c.sq_hi; t += kWave) dx += c.sqpart[t];
+            dot = wave_sum(dot);
+            yy = wave_sum(yy);
+            dx = wave_sum(dx);
+            if (lane == 0) {
+                out3[0] = dot;
+                out3[1] = yy;
+                out3[2] = dx;
+            }
+        }
+    }
+}
+
+}  // namespace

@@ -56,6 +56,8 @@ class SolverState(ctypes.Structure):
         ("pending", ctypes.c_int32),
         ("spare0", ctypes.c_int32),
         ("spare1", ctypes.c_int32),
+        ("step_budget", ctypes.c_int64),
+        ("reserved", ctypes.c_int64),
     ]
 
     def as_dict(self) -> dict:
@@ -106,6 +108,26 @@ SIGNATURES = {
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
     "pdot_copy_probe": (ctypes.c_int, [_ctx_p, ctypes.c_int64, ctypes.c_int32, _c_double_p]),
+    # batches
+    "pdot_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, ctypes.c_int32]),
+    "pdot_batch_destroy": (None, [_ctx_p]),
+    "pdot_batch_last_error": (ctypes.c_char_p, [_ctx_p]),
+    "pdot_batch_upload": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p]),
+    "pdot_batch_set_start": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, _c_double_p, _c_double_p]),
+    "pdot_batch_set_warm_start": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p, _c_double_p]),
+    "pdot_batch_set_active": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32)]),
+    "pdot_batch_run_steps": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(SolverState)]),
+    "pdot_batch_eval": (ctypes.c_int, [_ctx_p, ctypes.POINTER(PointStats), ctypes.POINTER(PointStats), ctypes.POINTER(SolverState)]),
+    "pdot_batch_restart": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), _c_double_p]),
+    "pdot_batch_discard": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32)]),
+    "pdot_batch_get_states": (ctypes.c_int, [_ctx_p, ctypes.POINTER(SolverState)]),
+    "pdot_batch_set_states": (ctypes.c_int, [_ctx_p, ctypes.POINTER(SolverState)]),
+    "pdot_batch_download": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p]),
+    "pdot_batch_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
+    "pdot_batch_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_batch_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
+    "pdot_batch_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
+    "pdot_batch_device_bytes": (ctypes.c_int64, [_ctx_p]),
 }
 
 _lib = None
@@ -249,3 +271,129 @@ class DeviceSolver:
         ms = ctypes.c_double()
         self._check(self.lib.pdot_copy_probe(self.h, int(nbytes), int(reps), ctypes.byref(ms)), "pdot_copy_probe")
         return ms.value
+
+
+def _i32(a):
+    a = np.ascontiguousarray(a, dtype=np.int32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+
+
+def _i64(a):
+    a = np.ascontiguousarray(a, dtype=np.int64)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+class BatchSolver:
+    """One pdot_batch: many problems resident in HBM, advanced together."""
+
+    def __init__(self, ns, ms, storage: int = STORAGE_F32, device: int = 0):
+        self.lib = load_library()
+        self.ns = [int(x) for x in ns]
+        self.ms = [int(x) for x in ms]
+        self.count = len(self.ns)
+        if self.count == 0 or len(self.ms) != self.count:
+            raise ValueError("need the same, non-zero number of row and column sizes")
+        self.storage = int(storage)
+        na, npp = _i64(self.ns)
+        ma, mp = _i64(self.ms)
+        h = ctypes.c_void_p()
+        rc = self.lib.pdot_batch_create(ctypes.byref(h), self.count, npp, mp, self.storage, int(device))
+        self.h = h
+        if rc != PDOT_OK:
+            msg = self.lib.pdot_batch_last_error(h).decode() if h.value else f"code {rc}"
+            if h.value:
+                self.lib.pdot_batch_destroy(h)
+            self.h = ctypes.c_void_p()
+            raise PdotError(f"pdot_batch_create failed ({rc}): {msg}")
+
+    def _check(self, rc: int, what: str) -> None:
+        if rc != PDOT_OK:
+            raise PdotError(f"{what} failed ({rc}): {self.lib.pdot_batch_last_error(self.h).decode()}")
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            self.lib.pdot_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, k: int, C, a, b) -> None:
+        C = np.asfortranarray(C, dtype=np.float64)
+        if C.shape != (self.ns[k], self.ms[k]):
+            raise ValueError(f"instance {k}: cost matrix shape {C.shape} != {(self.ns[k], self.ms[k])}")
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        self._check(self.lib.pdot_batch_upload(self.h, int(k), C.ctypes.data_as(_c_double_p), _ptr(a), _ptr(b)),
+                    "pdot_batch_upload")
+
+    def set_start(self, policy, reduction_exponent, growth_exponent, step_size, primal_weight) -> None:
+        eta = np.ascontiguousarray(np.broadcast_to(np.asarray(step_size, dtype=np.float64), (self.count,)))
+        om = np.ascontiguousarray(np.broadcast_to(np.asarray(primal_weight, dtype=np.float64), (self.count,)))
+        self._check(self.lib.pdot_batch_set_start(self.h, int(policy), float(reduction_exponent),
+                                                  float(growth_exponent), _ptr(eta), _ptr(om)), "pdot_batch_set_start")
+
+    def set_warm_start(self, k: int, u0, v0) -> None:
+        u0 = np.ascontiguousarray(u0, dtype=np.float64)
+        v0 = np.ascontiguousarray(v0, dtype=np.float64)
+        self._check(self.lib.pdot_batch_set_warm_start(self.h, int(k), _ptr(u0), _ptr(v0)), "pdot_batch_set_warm_start")
+
+    def set_active(self, active) -> None:
+        a, p = _i32(active)
+        self._check(self.lib.pdot_batch_set_active(self.h, p), "pdot_batch_set_active")
+
+    def run_steps(self, counts):
+        c, p = _i64(counts)
+        st = (SolverState * self.count)()
+        self._check(self.lib.pdot_batch_run_steps(self.h, p, st), "pdot_batch_run_steps")
+        return st
+
+    def eval(self):
+        cur, avg, st = (PointStats * self.count)(), (PointStats * self.count)(), (SolverState * self.count)()
+        self._check(self.lib.pdot_batch_eval(self.h, cur, avg, st), "pdot_batch_eval")
+        return cur, avg, st
+
+    def restart(self, actions, weights) -> None:
+        a, p = _i32(actions)
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        self._check(self.lib.pdot_batch_restart(self.h, p, _ptr(w)), "pdot_batch_restart")
+
+    def discard(self, which) -> None:
+        a, p = _i32(which)
+        self._check(self.lib.pdot_batch_discard(self.h, p), "pdot_batch_discard")
+
+    def get_states(self):
+        st = (SolverState * self.count)()
+        self._check(self.lib.pdot_batch_get_states(self.h, st), "pdot_batch_get_states")
+        return st
+
+    def download(self, k: int, which: int = POINT_CURRENT, want_x: bool = True):
+        n, m = self.ns[k], self.ms[k]
+        X = np.empty((n, m), dtype=np.float64, order="F") if want_x else None
+        u, v = np.empty(n), np.empty(m)
+        xp = X.ctypes.data_as(_c_double_p) if X is not None else None
+        self._check(self.lib.pdot_batch_download(self.h, int(k), int(which), xp, _ptr(u), _ptr(v)), "pdot_batch_download")
+        return X, u, v
+
+    def time_steps(self, count: int):
+        ks, tt = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.pdot_batch_time_steps(self.h, int(count), ctypes.byref(ks), ctypes.byref(tt)),
+                    "pdot_batch_time_steps")
+        return ks.value, tt.value
+
+    def use_graphs(self, enable: bool) -> None:
+        self._check(self.lib.pdot_batch_use_graphs(self.h, int(bool(enable))), "pdot_batch_use_graphs")
+
+    def set_launch(self, cols_per_tile: int, grid_mult: int = 1) -> None:
+        self._check(self.lib.pdot_batch_set_launch(self.h, int(cols_per_tile), int(grid_mult)), "pdot_batch_set_launch")
+
+    def get_launch(self) -> dict:
+        vals = [ctypes.c_int32() for _ in range(4)]
+        self._check(self.lib.pdot_batch_get_launch(self.h, *[ctypes.byref(v) for v in vals]), "pdot_batch_get_launch")
+        return dict(zip(("rows_per_tile", "cols_per_tile", "tiles", "step_grid"), (v.value for v in vals)))
+
+    def device_bytes(self) -> int:
+        return int(self.lib.pdot_batch_device_bytes(self.h))

@@ -175,6 +175,85 @@ class SolveProfile:
     take_step_calls: int = 0
 
 
+def initial_step_and_weight(params: PdhgParameters, nrow: int, ncol: int, ot_cache: CachedOTInfo):
+    """Initial step size, primal weight and step policy (solver_gpu.jl:476-496)."""
+    sp = params.step_size_policy_params
+    if isinstance(sp, AdaptiveStepsizeParams):
+        policy, re, ge = _lib.POLICY_ADAPTIVE, float(sp.reduction_exponent), float(sp.growth_exponent)
+        step_size = 1.0
+    else:
+        policy, re, ge = _lib.POLICY_CONSTANT, 0.0, 0.0
+        desired_relative_error = 0.2
+        step_size = (1 - desired_relative_error) / math.sqrt(nrow + ncol)
+    if params.scale_invariant_initial_primal_weight:
+        primal_weight = select_initial_primal_weight(ot_cache.l2_norm_primal_linear_objective,
+                                                     ot_cache.l2_norm_primal_right_hand_side, 1.0, 1.0,
+                                                     params.primal_importance, params.verbosity)
+    else:
+        primal_weight = params.primal_importance
+    return policy, re, ge, step_size, primal_weight
+
+
+class InstanceController:
+    """The host half of one solve: at each evaluation, the statistics,
+    termination check and restart decision of optimize (solver_gpu.jl:529-696)
+    from the device reductions of the current point and its average."""
+
+    KKT_PASSES_PER_TERMINATION_EVALUATION = 2.0
+
+    def __init__(self, params: PdhgParameters, ot_cache: CachedOTInfo):
+        self.params = params
+        self.ot_cache = ot_cache
+        crit = params.termination_criteria
+        self.eps_ratio = crit.eps_optimal_absolute / crit.eps_optimal_relative
+        self.eval_passes = 0.0
+        self.last_restart_info = None
+        self.iteration_stats: List[IterationStats] = []
+        self.evaluations = 0
+        self.restarts = 0
+
+    def evaluate(self, cur, avg, st, iteration: int, elapsed: float, time_basic: float):
+        """Returns (termination_reason or None, stats)."""
+        self.eval_passes += self.KKT_PASSES_PER_TERMINATION_EVALUATION
+        self.evaluations += 1
+        if self.last_restart_info is None:
+            self.last_restart_info = create_last_restart_info(KKTPieces.from_point_stats(cur))
+        numerical_error = bool(st.numerical_error)
+        point = cur if (numerical_error or st.avg_count == 0) else avg
+        stats = IterationStats()
+        stats.iteration_number = iteration
+        stats.cumulative_kkt_matrix_passes = st.cumulative_kkt_passes + self.eval_passes
+        stats.cumulative_time_sec = elapsed
+        stats.convergence_information = [convergence_information(point, self.ot_cache, self.eps_ratio,
+                                                                 PointType.POINT_TYPE_AVERAGE_ITERATE)]
+        stats.step_size = st.step_size
+        stats.primal_weight = st.primal_weight
+        stats.method_specific_stats = {"time_spent_doing_basic_algorithm": time_basic}
+        reason = check_termination_criteria(self.params.termination_criteria, self.ot_cache, stats)
+        if numerical_error and reason is None:
+            reason = TerminationReason.TERMINATION_REASON_NUMERICAL_ERROR
+        if self.params.record_iteration_stats or reason is not None:
+            self.iteration_stats.append(stats)
+        return reason, stats
+
+    def restart(self, cur, avg, st, iteration: int, stats: IterationStats):
+        """run_restart_scheme + compute_new_primal_weight; returns
+        (decision, new primal weight)."""
+        primal_norm_params, dual_norm_params = define_norms(st.step_size, st.primal_weight)
+        rp = self.params.restart_params
+        decision = run_restart_scheme(int(st.avg_count), KKTPieces.from_point_stats(cur),
+                                      KKTPieces.from_point_stats(avg) if st.avg_count > 0 else None,
+                                      self.last_restart_info, iteration - 1, primal_norm_params, dual_norm_params,
+                                      st.primal_weight, rp)
+        stats.restart_used = decision.choice
+        weight = st.primal_weight
+        if decision.choice != RestartChoice.RESTART_CHOICE_NO_RESTART:
+            self.restarts += 1
+            weight = compute_new_primal_weight(self.last_restart_info, st.primal_weight,
+                                               rp.primal_weight_update_smoothing, self.params.verbosity)
+        return decision, weight
+
+
 def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, *,
              storage: int = _lib.STORAGE_F64, device: int = 0, warm_start=None,
              solver: Optional[_lib.DeviceSolver] = None, profile: Optional[SolveProfile] = None,
@@ -203,99 +282,48 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
     solver.use_graphs(use_graphs)
     upload_sec = time.time() - t_up
 
-    sp = params.step_size_policy_params
-    if isinstance(sp, AdaptiveStepsizeParams):
-        policy, re, ge = _lib.POLICY_ADAPTIVE, float(sp.reduction_exponent), float(sp.growth_exponent)
-        step_size = 1.0
-    else:
-        policy, re, ge = _lib.POLICY_CONSTANT, 0.0, 0.0
-        desired_relative_error = 0.2
-        step_size = (1 - desired_relative_error) / math.sqrt(nrow + ncol)
-    KKT_PASSES_PER_TERMINATION_EVALUATION = 2.0
-    if params.scale_invariant_initial_primal_weight:
-        primal_weight = select_initial_primal_weight(ot_cache.l2_norm_primal_linear_objective,
-                                                     ot_cache.l2_norm_primal_right_hand_side, 1.0, 1.0,
-                                                     params.primal_importance, params.verbosity)
-    else:
-        primal_weight = params.primal_importance
-    restart_params = params.restart_params
-    smoothing = restart_params.primal_weight_update_smoothing
+    policy, re, ge, step_size, primal_weight = initial_step_and_weight(params, nrow, ncol, ot_cache)
     u0 = v0 = None
     if warm_start is not None:
         u0, v0 = warm_start
     solver.set_start(u0, v0, policy, re, ge, step_size, primal_weight)
 
-    iteration_stats: List[IterationStats] = []
+    ctl = InstanceController(params, ot_cache)
     start_time = time.time()
     time_basic = 0.0
-    criteria = params.termination_criteria
-    iteration_limit = criteria.iteration_limit
+    iteration_limit = params.termination_criteria.iteration_limit
     freq = params.termination_evaluation_frequency
-    eps_ratio = criteria.eps_optimal_absolute / criteria.eps_optimal_relative
     if params.verbosity >= 2:
         display_iteration_stats_heading()
 
-    last_restart_info = None
-    eval_passes = 0.0
-    n_eval = n_restart = 0
     iteration = 1
     while True:
         # `iteration` is an evaluation iteration: evaluate the state before its step
-        eval_passes += KKT_PASSES_PER_TERMINATION_EVALUATION
         t0 = time.time()
         cur, avg, st = solver.eval()
         time_basic += time.time() - t0
-        n_eval += 1
-        if last_restart_info is None:
-            last_restart_info = create_last_restart_info(KKTPieces.from_point_stats(cur))
-        numerical_error = bool(st.numerical_error)
-        use_current = numerical_error or st.avg_count == 0
-        point = cur if use_current else avg
-        stats = IterationStats()
-        stats.iteration_number = iteration
-        stats.cumulative_kkt_matrix_passes = st.cumulative_kkt_passes + eval_passes
-        stats.cumulative_time_sec = time.time() - start_time
-        stats.convergence_information = [convergence_information(point, ot_cache, eps_ratio,
-                                                                 PointType.POINT_TYPE_AVERAGE_ITERATE)]
-        stats.step_size = st.step_size
-        stats.primal_weight = st.primal_weight
-        stats.method_specific_stats = {"time_spent_doing_basic_algorithm": time_basic}
-        primal_norm_params, dual_norm_params = define_norms(st.step_size, st.primal_weight)
-
-        termination_reason = check_termination_criteria(criteria, ot_cache, stats)
-        if numerical_error and termination_reason is None:
-            termination_reason = TerminationReason.TERMINATION_REASON_NUMERICAL_ERROR
-        if params.record_iteration_stats or termination_reason is not None:
-            iteration_stats.append(stats)
-        if print_to_screen_this_iteration(termination_reason, iteration, params.verbosity, freq):
+        reason, stats = ctl.evaluate(cur, avg, st, iteration, time.time() - start_time, time_basic)
+        if print_to_screen_this_iteration(reason, iteration, params.verbosity, freq):
             display_iteration_stats(stats)
-
-        if termination_reason is not None:
+        if reason is not None:
             solver.discard()
             which = _lib.POINT_CURRENT if st.avg_count == 0 else _lib.POINT_AVERAGE
             X, u, v = solver.download(which)
             if profile is not None:
                 profile.upload_sec = upload_sec
                 profile.solve_sec = time.time() - start_time
-                profile.evaluations = n_eval
-                profile.restarts = n_restart
+                profile.evaluations = ctl.evaluations
+                profile.restarts = ctl.restarts
                 profile.take_step_calls = int(st.take_step_calls)
             if params.verbosity >= 1:
-                print(f"Terminated after {iteration} iterations: {termination_reason_to_string(termination_reason)}")
-            return SaddlePointOutput(X, u, v, termination_reason, termination_reason_to_string(termination_reason),
-                                     iteration - 1, iteration_stats)
-
-        decision = run_restart_scheme(int(st.avg_count), KKTPieces.from_point_stats(cur),
-                                      KKTPieces.from_point_stats(avg) if st.avg_count > 0 else None,
-                                      last_restart_info, iteration - 1, primal_norm_params, dual_norm_params,
-                                      st.primal_weight, restart_params)
-        stats.restart_used = decision.choice
+                print(f"Terminated after {iteration} iterations: {termination_reason_to_string(reason)}")
+            return SaddlePointOutput(X, u, v, reason, termination_reason_to_string(reason), iteration - 1,
+                                     ctl.iteration_stats)
+        decision, weight = ctl.restart(cur, avg, st, iteration, stats)
         t0 = time.time()
         nxt = next_evaluation_iteration(iteration, freq, iteration_limit)
         if decision.choice != RestartChoice.RESTART_CHOICE_NO_RESTART:
-            n_restart += 1
-            new_weight = compute_new_primal_weight(last_restart_info, st.primal_weight, smoothing, params.verbosity)
-            solver.restart(decision.to_average, new_weight)
+            solver.restart(decision.to_average, weight)
             count = nxt - iteration  # the speculative step was discarded: redo it
         else:
             count = nxt - iteration - 1  # the speculative step is committed first

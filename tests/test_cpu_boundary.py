@@ -29,8 +29,8 @@ def test_binding_covers_header():
 
 
 def test_struct_sizes_match_header_layout():
-    # pdot_solver_state: 4 doubles, 4 int64, 8 doubles, 8 int32
-    assert ctypes.sizeof(_lib.SolverState) == 4 * 8 + 4 * 8 + 8 * 8 + 8 * 4
+    # pdot_solver_state: 4 doubles, 4 int64, 8 doubles, 8 int32, 2 int64
+    assert ctypes.sizeof(_lib.SolverState) == 4 * 8 + 4 * 8 + 8 * 8 + 8 * 4 + 2 * 8
     assert ctypes.sizeof(_lib.PointStats) == 11 * 8
 
 
