@@ -222,6 +222,34 @@ int pdot_batch_get_launch(pdot_batch* b, int32_t* rows_per_tile, int32_t* cols_p
                           int32_t* step_grid);
 int64_t pdot_batch_device_bytes(const pdot_batch* b);
 
+/* ------------------------------------------------------------------------
+ * Learned warm start: a point-wise network with mean-pooled set context that
+ * predicts dual potentials (u, v) from point coordinates and masses.  No
+ * counterpart in the reference; it only chooses the starting iterate handed
+ * to pdot_set_start / pdot_batch_set_warm_start.  All matrix products run on
+ * bf16 MFMA with FP32 accumulation.  Features are N x 4 row-major FP32 rows
+ * (x, y, scaled mass, side), the nsrc source rows first; the parameter
+ * vector layout is documented in pdot_mlp.hip.
+ * ------------------------------------------------------------------------ */
+typedef struct pdot_mlp pdot_mlp;
+
+int pdot_mlp_create(pdot_mlp** out, int32_t device);
+void pdot_mlp_destroy(pdot_mlp* m);
+const char* pdot_mlp_last_error(const pdot_mlp* m);
+int64_t pdot_mlp_num_params(const pdot_mlp* m);
+int pdot_mlp_set_params(pdot_mlp* m, const float* params);  /* also resets the Adam moments */
+int pdot_mlp_get_params(pdot_mlp* m, float* params);
+int pdot_mlp_forward(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, float* out);
+/* Mean squared error against target (N) and its gradient (num_params, may
+ * be NULL); the gradient stays on the device for pdot_mlp_adam_step. */
+int pdot_mlp_loss_grad(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, const float* target,
+                       float* loss, float* grad);
+int pdot_mlp_adam_step(pdot_mlp* m, float lr, float beta1, float beta2, float eps);
+/* Test hook for the MFMA matrix product: C (M x N) = op(A) op(B), operands
+ * rounded to bf16, FP32 accumulation; row-major, op = transpose if set. */
+int pdot_mlp_gemm_probe(pdot_mlp* m, const float* A, const float* B, float* C, int64_t M, int64_t N, int64_t K,
+                        int32_t trans_a, int32_t trans_b);
+
 #ifdef __cplusplus
 }
 #endif

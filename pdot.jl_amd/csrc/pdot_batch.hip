@@ -39,6 +39,8 @@ constexpr int kBWaves = 4;                 // batched step kernel: waves per blo
 constexpr int kBBlock = kWave * kBWaves;
 constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block
 constexpr int kBRedBlock = kWave * kBRedWaves;
+constexpr int kMinColTiles = 32;           // column tiles per instance (at least)
+constexpr int kMinTileCols = 4;            // columns per tile (at least)
 
 struct BInst {
     int64_t n, m, ld;
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             a.gzero = sst.gzero != 0;
             a.do_step = do_step;
             double dd = 0.0, ev[kMatStats];
-            step_tile<T, EVAL, 1, kBWaves>(a, colbuf, red, &dd, ev);
+            step_tile<T, EVAL, 2, kBWaves>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
                 const int64_t lb = stl.rb + (int64_t)stl.cb * sinst.R;
                 if (do_step) p.part[sinst.sqpart + lb] = dd;
@@ -386,7 +388,11 @@ int64_t layout_partials(pdot_batch* b) {
     b->max_chunks = 0;
     for (auto& in : b->inst) {
         in.R = (int32_t)((in.n + TR - 1) / TR);
-        in.TC = b->cols_per_tile;
+        // at least kMinColTiles column tiles per instance, so that a small
+        // instance left alone in the batch is spread over many blocks instead
+        // of one block walking its columns one load latency at a time
+        in.TC = (int32_t)std::min<int64_t>(b->cols_per_tile,
+                                           std::max<int64_t>(kMinTileCols, (in.m + kMinColTiles - 1) / kMinColTiles));
         in.Cb = (int32_t)((in.m + in.TC - 1) / in.TC);
         in.nstep = in.R * in.Cb;
         in.NBr = (int32_t)((in.n + kWave - 1) / kWave);
@@ -415,7 +421,7 @@ int rebuild_work(pdot_batch* b) {
         if (!b->active[k]) continue;
         const BInst& in = b->inst[k];
         for (int32_t cb = 0; cb < in.Cb; ++cb) {
-            const int64_t cols = std::min<int64_t>(in.TC, in.m - (int64_t)cb * in.TC);
+            const int64_t cols = std::min<int64_t>(in.TC, in.m - (int64_t)cb * in.TC);  // tile work for the sort
             for (int32_t rb = 0; rb < in.R; ++rb) {
                 const int64_t rows = std::min<int64_t>(TR, in.n - (int64_t)rb * TR);
                 weighted.push_back({rows * cols, BTile{k, rb, cb, 0}});

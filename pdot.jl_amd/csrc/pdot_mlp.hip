@@ -1,0 +1,480 @@
+// pdot_mlp.hip -- the learned warm start's network on gfx950: a point-wise
+// MLP with a mean-pooled set context that maps (point coordinates, scaled
+// mass, side) to a dual potential, trained and evaluated with hand-written
+// kernels.  Every matrix product runs on MFMA (v_mfma_f32_32x32x16_bf16:
+// bf16 operands rounded to nearest even from FP32, FP32 accumulation); the
+// small vector work (bias, ReLU, pooling, the context bias, loss, Adam) is
+// plain FP32.
+//
+// Network (row-major weights, Y = X W), for N = ns + nt points (source rows
+// first):
+//   Z1 = X0 W1 + b1,          H1 = relu(Z1)                 X0: N x 4 features
+//   cs = mean(H1[source]),    ct = mean(H1[target])          (1 x 128 each)
+//   g  = cs W2b + ct W2c + b2                                 context bias
+//   Z2 = H1 W2a + g,          H2 = relu(Z2)
+//   Z3 = H2 W3 + b3,          H3 = relu(Z3)
+//   out = H3 w4 + b4                                           N x 1
+// Loss: mean((out - target)^2).  Parameters (flat FP32, this order):
+//   W1 4x128, b1 128, W2a 128x128, W2b 128x128, W2c 128x128, b2 128,
+//   W3 128x128, b3 128, w4 128, b4 1.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "pdot.h"
+
+namespace {
+
+constexpr int kW = 128;   // hidden width
+constexpr int kF = 4;     // input features
+constexpr int kFP = 16;   // input features padded to the MFMA k-step
+constexpr int kT = 32;    // MFMA output tile (32 x 32)
+constexpr int kKs = 16;   // MFMA k-step
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+// FP32 -> bf16 bits, round to nearest even (NaN kept quiet).
+__device__ __forceinline__ unsigned short bf16_bits(float f) {
+    unsigned int u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return (unsigned short)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ __bf16 to_bf16(float f) {
+    unsigned short b = bf16_bits(f);
+    return __builtin_bit_cast(__bf16, b);
+}
+
+// C[M x N] = op(A) op(B) with op(A)[r][k] = TA ? A[k*lda + r] : A[r*lda + k]
+// and op(B)[k][c] = TB ? B[c*ldb + k] : B[k*ldb + c]; one wave per 32 x 32
+// output tile, k in steps of 16.  MFMA 32x32x16 operand layout: lane l holds
+// row (l % 32) of A and column (l % 32) of B, k = 8*(l / 32) + 0..7; the
+// 16 accumulators of lane l hold column (l % 32), rows 8*(r/4) + 4*(l/32) + r%4.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(64) void gemm_bf16_kernel(const float* __restrict__ A, int64_t lda,
+                                                       const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
+                                                       int64_t ldc, int64_t M, int64_t N, int64_t K) {
+    const int lane = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * kT, c0 = (int64_t)blockIdx.y * kT;
+    const int li = lane & 31, kh = (lane >> 5) * 8;
+    f32x16 acc;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[t] = 0.f;
+    for (int64_t k0 = 0; k0 < K; k0 += kKs) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int64_t r = r0 + li, c = c0 + li, k = k0 + kh + j;
+            float av = 0.f, bv = 0.f;
+            if (r < M && k < K) av = TA ? A[k * lda + r] : A[r * lda + k];
+            if (c < N && k < K) bv = TB ? B[c * ldb + k] : B[k * ldb + c];
+            a[j] = to_bf16(av);
+            b[j] = to_bf16(bv);
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        const int64_t r = r0 + 8 * (t / 4) + 4 * (lane >> 5) + (t % 4), c = c0 + li;
+        if (r < M && c < N) C[r * ldc + c] = acc[t];
+    }
+}
+
+// Z = Y + bias (+ row-broadcast), H = relu(Z); Y may alias Z.
+__global__ void bias_relu_kernel(const float* Y, const float* bias, float* Z, float* H, int64_t N, int w) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N * w) return;
+    const float z = Y[t] + bias[t % w];
+    Z[t] = z;
+    H[t] = z > 0.f ? z : 0.f;
+}
+
+// Column means of H over rows [0, ns) and [ns, ns + nt): cs, ct (w threads).
+__global__ void pool_kernel(const float* H, int64_t ns, int64_t nt, int w, float* cs, float* ct) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w) return;
+    float s = 0.f, t = 0.f;
+    for (int64_t i = 0; i < ns; ++i) s += H[i * w + c];
+    for (int64_t i = ns; i < ns + nt; ++i) t += H[i * w + c];
+    cs[c] = ns > 0 ? s / (float)ns : 0.f;
+    ct[c] = nt > 0 ? t / (float)nt : 0.f;
+}
+
+// g = cs W2b + ct W2c + b2 (w threads).
+__global__ void context_kernel(const float* cs, const float* ct, const float* W2b, const float* W2c, const float* b2,
+                               float* g, int w) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w) return;
+    float s = 0.f;
+    for (int k = 0; k < w; ++k) s += cs[k] * W2b[k * w + c];
+    float t = 0.f;
+    for (int k = 0; k < w; ++k) t += ct[k] * W2c[k * w + c];
+    g[c] = (s + t) + b2[c];
+}
+
+// out[i] = Y[i] + b4 (Y is column 0 of a width-ld product); dout = 2(out - t)/N.
+__global__ void head_loss_kernel(const float* Y, int64_t ldy, const float* b4, const float* target, float* out,
+                                 float* dout, float* sq, int64_t N) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const float o = Y[i * ldy] + b4[0];
+    out[i] = o;
+    if (target) {
+        const float d = o - target[i];
+        dout[i] = 2.f * d / (float)N;
+        sq[i] = d * d;
+    }
+}
+
+// dZ = dH * (Z > 0), in place on dH.
+__global__ void relu_back_kernel(float* dH, const float* Z, int64_t total) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < total && !(Z[t] > 0.f)) dH[t] = 0.f;
+}
+
+// Column sums of D (N x w) into out[w] (w threads; fixed row order).
+__global__ void colsum_kernel(const float* D, int64_t N, int w, float* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= w) return;
+    float s = 0.f;
+    for (int64_t i = 0; i < N; ++i) s += D[i * w + c];
+    out[c] = s;
+}
+
+// Context backward: with dg = colsum(dZ2): dW2b = cs^T dg, dW2c = ct^T dg,
+// dcs = W2b dg, dct = W2c dg; one thread per row k of W2b/W2c.
+__global__ void context_back_kernel(const float* cs, const float* ct, const float* W2b, const float* W2c,
+                                    const float* dg, float* dW2b, float* dW2c, float* dcs, float* dct, int w) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= w) return;
+    float s = 0.f, t = 0.f;
+    for (int c = 0; c < w; ++c) {
+        dW2b[k * w + c] = cs[k] * dg[c];
+        dW2c[k * w + c] = ct[k] * dg[c];
+        s += W2b[k * w + c] * dg[c];
+        t += W2c[k * w + c] * dg[c];
+    }
+    dcs[k] = s;
+    dct[k] = t;
+}
+
+// dH1 += dcs / ns on source rows and dct / nt on target rows.
+__global__ void pool_back_kernel(float* dH1, const float* dcs, const float* dct, int64_t ns, int64_t nt, int w) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (ns + nt) * w) return;
+    const int64_t i = t / w;
+    const int c = (int)(t % w);
+    dH1[t] += i < ns ? dcs[c] / (float)ns : dct[c] / (float)nt;
+}
+
+// Adam (bias-corrected), one thread per parameter.
+__global__ void adam_kernel(float* p, const float* g, float* m, float* v, int64_t np, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= np) return;
+    const float gt = g[t];
+    const float mt = b1 * m[t] + (1.f - b1) * gt;
+    const float vt = b2 * v[t] + (1.f - b2) * gt * gt;
+    m[t] = mt;
+    v[t] = vt;
+    p[t] -= lr * (mt / c1) / (sqrtf(vt / c2) + eps);
+}
+
+// Pad features N x 4 -> N x 16 (zeros), and W1 4 x 128 -> 16 x 128.
+__global__ void pad_kernel(const float* src, int64_t rows, int cols, float* dst, int dcols) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * dcols) return;
+    const int64_t i = t / dcols;
+    const int c = (int)(t % dcols);
+    dst[t] = c < cols ? src[i * cols + c] : 0.f;
+}
+
+unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+struct pdot_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // parameter offsets in the flat vector
+    int64_t oW1, ob1, oW2a, oW2b, oW2c, ob2, oW3, ob3, ow4, ob4, np;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // params, grads, Adam moments
+    int64_t adam_t = 0;
+    // activations for the current N
+    int64_t cap = 0;
+    float *X0 = nullptr, *Z1 = nullptr, *H1 = nullptr, *Z2 = nullptr, *H2 = nullptr, *Z3 = nullptr, *H3 = nullptr;
+    float *Y4 = nullptr, *out = nullptr, *tgt = nullptr, *dout = nullptr, *sq = nullptr, *D = nullptr, *E = nullptr;
+    float *feat = nullptr;
+    float *W1p = nullptr, *dW1p = nullptr, *w4p = nullptr;  // padded W1 (16x128), its gradient, w4 as 128 x 32
+    float *cs = nullptr, *ct = nullptr, *gctx = nullptr, *dcs = nullptr, *dct = nullptr, *dg = nullptr;
+    std::vector<float> hbuf;
+    std::string err;
+};
+
+namespace {
+
+int mfail(pdot_mlp* m, int code, const std::string& msg) {
+    if (m) m->err = msg;
+    return code;
+}
+#define MHIPCHK(mm, call)                                                                               \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess) return mfail((mm), PDOT_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+int malloc_f(pdot_mlp* m, float** p, int64_t n) {
+    hipError_t e = hipMalloc((void**)p, sizeof(float) * (size_t)(n > 0 ? n : 1));
+    if (e != hipSuccess) return mfail(m, PDOT_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    return PDOT_OK;
+}
+
+int ensure_cap(pdot_mlp* m, int64_t N) {
+    if (N <= m->cap) return PDOT_OK;
+    float** bufs[] = {&m->X0, &m->Z1, &m->H1, &m->Z2, &m->H2, &m->Z3, &m->H3, &m->D, &m->E};
+    for (float** b : bufs)
+        if (*b) (void)hipFree(*b);
+    float** vecs[] = {&m->Y4, &m->out, &m->tgt, &m->dout, &m->sq};
+    for (float** b : vecs)
+        if (*b) (void)hipFree(*b);
+    if (m->feat) (void)hipFree(m->feat);
+    int rc;
+    if ((rc = malloc_f(m, &m->X0, N * kFP))) return rc;
+    float** wide[] = {&m->Z1, &m->H1, &m->Z2, &m->H2, &m->Z3, &m->H3, &m->D, &m->E};
+    for (float** b : wide)
+        if ((rc = malloc_f(m, b, N * kW))) return rc;
+    if ((rc = malloc_f(m, &m->Y4, N * kT))) return rc;
+    float** narrow[] = {&m->out, &m->tgt, &m->dout, &m->sq};
+    for (float** b : narrow)
+        if ((rc = malloc_f(m, b, N))) return rc;
+    if ((rc = malloc_f(m, &m->feat, N * kF))) return rc;
+    m->cap = N;
+    return PDOT_OK;
+}
+
+template <bool TA, bool TB>
+hipError_t gemm(pdot_mlp* m, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
+                int64_t M, int64_t N, int64_t K) {
+    dim3 grid((unsigned)((M + kT - 1) / kT), (unsigned)((N + kT - 1) / kT));
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(64), 0, m->stream, A, lda, B, ldb, C, ldc, M, N, K);
+    return hipGetLastError();
+}
+
+// Forward pass on the features already in m->feat; fills all activations.
+int forward(pdot_mlp* m, int64_t ns, int64_t nt, const float* target_dev) {
+    const int64_t N = ns + nt;
+    float* P = m->p;
+    hipLaunchKernelGGL(pad_kernel, dim3(blocks_for(N * kFP)), dim3(256), 0, m->stream, m->feat, N, kF, m->X0, kFP);
+    // W1 (4 x 128) as a 16 x 128 matrix with zero rows 4..15
+    MHIPCHK(m, hipMemsetAsync(m->W1p, 0, sizeof(float) * kFP * kW, m->stream));
+    MHIPCHK(m, hipMemcpyAsync(m->W1p, P + m->oW1, sizeof(float) * kF * kW, hipMemcpyDeviceToDevice, m->stream));
+    MHIPCHK(m, (gemm<false, false>(m, m->X0, kFP, m->W1p, kW, m->Z1, kW, N, kW, kFP)));
+    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->Z1, P + m->ob1, m->Z1,
+                       m->H1, N, kW);
+    hipLaunchKernelGGL(pool_kernel, dim3(1), dim3(kW), 0, m->stream, m->H1, ns, nt, kW, m->cs, m->ct);
+    hipLaunchKernelGGL(context_kernel, dim3(1), dim3(kW), 0, m->stream, m->cs, m->ct, P + m->oW2b, P + m->oW2c,
+                       P + m->ob2, m->gctx, kW);
+    MHIPCHK(m, (gemm<false, false>(m, m->H1, kW, P + m->oW2a, kW, m->Z2, kW, N, kW, kW)));
+    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->Z2, m->gctx, m->Z2,
+                       m->H2, N, kW);
+    MHIPCHK(m, (gemm<false, false>(m, m->H2, kW, P + m->oW3, kW, m->Z3, kW, N, kW, kW)));
+    hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->Z3, P + m->ob3, m->Z3,
+                       m->H3, N, kW);
+    // w4 (128) as a 128 x 32 matrix with zero columns 1..31
+    MHIPCHK(m, hipMemsetAsync(m->w4p, 0, sizeof(float) * kW * kT, m->stream));
+    MHIPCHK(m, hipMemcpy2DAsync(m->w4p, sizeof(float) * kT, P + m->ow4, sizeof(float), sizeof(float), kW,
+                                hipMemcpyDeviceToDevice, m->stream));
+    MHIPCHK(m, (gemm<false, false>(m, m->H3, kW, m->w4p, kT, m->Y4, kT, N, kT, kW)));
+    hipLaunchKernelGGL(head_loss_kernel, dim3(blocks_for(N)), dim3(256), 0, m->stream, m->Y4, (int64_t)kT,
+                       P + m->ob4, target_dev, m->out, m->dout, m->sq, N);
+    MHIPCHK(m, hipGetLastError());
+    return PDOT_OK;
+}
+
+// Backward pass after forward(...) with a target: fills m->g.
+int backward(pdot_mlp* m, int64_t ns, int64_t nt) {
+    const int64_t N = ns + nt;
+    float* P = m->p;
+    float* G = m->g;
+    // head: dW4 = H3^T dout (128 x 1), db4 = sum dout, dH3 = dout w4^T
+    MHIPCHK(m, (gemm<true, false>(m, m->H3, kW, m->dout, 1, G + m->ow4, 1, kW, 1, N)));
+    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1), 0, m->stream, m->dout, N, 1, G + m->ob4);
+    MHIPCHK(m, (gemm<false, true>(m, m->dout, 1, P + m->ow4, 1, m->D, kW, N, kW, 1)));
+    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->D, m->Z3, N * kW);
+    // layer 3: dW3 = H2^T dZ3, db3 = colsum dZ3, dH2 = dZ3 W3^T
+    MHIPCHK(m, (gemm<true, false>(m, m->H2, kW, m->D, kW, G + m->oW3, kW, kW, kW, N)));
+    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, G + m->ob3);
+    MHIPCHK(m, (gemm<false, true>(m, m->D, kW, P + m->oW3, kW, m->E, kW, N, kW, kW)));
+    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->E, m->Z2, N * kW);
+    // layer 2: dW2a = H1^T dZ2, dg = colsum dZ2 (= db2), context weights, dH1 = dZ2 W2a^T + pooling
+    MHIPCHK(m, (gemm<true, false>(m, m->H1, kW, m->E, kW, G + m->oW2a, kW, kW, kW, N)));
+    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->E, N, kW, G + m->ob2);
+    hipLaunchKernelGGL(context_back_kernel, dim3(1), dim3(kW), 0, m->stream, m->cs, m->ct, P + m->oW2b, P + m->oW2c,
+                       G + m->ob2, G + m->oW2b, G + m->oW2c, m->dcs, m->dct, kW);
+    MHIPCHK(m, (gemm<false, true>(m, m->E, kW, P + m->oW2a, kW, m->D, kW, N, kW, kW)));
+    hipLaunchKernelGGL(pool_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->D, m->dcs, m->dct, ns,
+                       nt, kW);
+    hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->D, m->Z1, N * kW);
+    // layer 1: dW1 = X0^T dZ1 (16 x 128, rows 0..3 kept), db1 = colsum dZ1
+    MHIPCHK(m, (gemm<true, false>(m, m->X0, kFP, m->D, kW, m->dW1p, kW, kFP, kW, N)));
+    MHIPCHK(m, hipMemcpyAsync(G + m->oW1, m->dW1p, sizeof(float) * kF * kW, hipMemcpyDeviceToDevice, m->stream));
+    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, G + m->ob1);
+    MHIPCHK(m, hipGetLastError());
+    return PDOT_OK;
+}
+
+int upload_inputs(pdot_mlp* m, const float* feats, int64_t N, const float* target) {
+    int rc = ensure_cap(m, N);
+    if (rc) return rc;
+    MHIPCHK(m, hipMemcpyAsync(m->feat, feats, sizeof(float) * N * kF, hipMemcpyHostToDevice, m->stream));
+    if (target) MHIPCHK(m, hipMemcpyAsync(m->tgt, target, sizeof(float) * N, hipMemcpyHostToDevice, m->stream));
+    return PDOT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pdot_mlp_create(pdot_mlp** out, int32_t device) {
+    if (!out) return PDOT_E_ARG;
+    pdot_mlp* m = new pdot_mlp();
+    *out = m;
+    m->device = device;
+    MHIPCHK(m, hipSetDevice(device));
+    MHIPCHK(m, hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+    int64_t o = 0;
+    m->oW1 = o; o += kF * kW;
+    m->ob1 = o; o += kW;
+    m->oW2a = o; o += kW * kW;
+    m->oW2b = o; o += kW * kW;
+    m->oW2c = o; o += kW * kW;
+    m->ob2 = o; o += kW;
+    m->oW3 = o; o += kW * kW;
+    m->ob3 = o; o += kW;
+    m->ow4 = o; o += kW;
+    m->ob4 = o; o += 1;
+    m->np = o;
+    int rc;
+    float** pv[] = {&m->p, &m->g, &m->m, &m->v};
+    for (float** q : pv) {
+        if ((rc = malloc_f(m, q, m->np))) return rc;
+        MHIPCHK(m, hipMemsetAsync(*q, 0, sizeof(float) * m->np, m->stream));
+    }
+    if ((rc = malloc_f(m, &m->W1p, kFP * kW))) return rc;
+    if ((rc = malloc_f(m, &m->dW1p, kFP * kW))) return rc;
+    if ((rc = malloc_f(m, &m->w4p, kW * kT))) return rc;
+    float** sm[] = {&m->cs, &m->ct, &m->gctx, &m->dcs, &m->dct, &m->dg};
+    for (float** q : sm)
+        if ((rc = malloc_f(m, q, kW))) return rc;
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return PDOT_OK;
+}
+
+void pdot_mlp_destroy(pdot_mlp* m) {
+    if (!m) return;
+    if (m->stream) (void)hipStreamSynchronize(m->stream);
+    float* all[] = {m->p, m->g, m->m, m->v, m->X0, m->Z1, m->H1, m->Z2, m->H2, m->Z3, m->H3, m->Y4, m->out, m->tgt,
+                    m->dout, m->sq, m->D, m->E, m->feat, m->W1p, m->dW1p, m->w4p, m->cs, m->ct, m->gctx, m->dcs,
+                    m->dct, m->dg};
+    for (float* q : all)
+        if (q) (void)hipFree(q);
+    if (m->stream) (void)hipStreamDestroy(m->stream);
+    delete m;
+}
+
+const char* pdot_mlp_last_error(const pdot_mlp* m) { return m ? m->err.c_str() : "null network"; }
+
+int64_t pdot_mlp_num_params(const pdot_mlp* m) { return m ? m->np : 0; }
+
+int pdot_mlp_set_params(pdot_mlp* m, const float* params) {
+    if (!m || !params) return PDOT_E_ARG;
+    MHIPCHK(m, hipMemcpyAsync(m->p, params, sizeof(float) * m->np, hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipMemsetAsync(m->m, 0, sizeof(float) * m->np, m->stream));
+    MHIPCHK(m, hipMemsetAsync(m->v, 0, sizeof(float) * m->np, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    m->adam_t = 0;
+    return PDOT_OK;
+}
+
+int pdot_mlp_get_params(pdot_mlp* m, float* params) {
+    if (!m || !params) return PDOT_E_ARG;
+    MHIPCHK(m, hipMemcpyAsync(params, m->p, sizeof(float) * m->np, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return PDOT_OK;
+}
+
+int pdot_mlp_forward(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, float* out) {
+    if (!m || !features || !out || nsrc < 0 || ntgt < 0 || nsrc + ntgt == 0) return PDOT_E_ARG;
+    const int64_t N = nsrc + ntgt;
+    int rc = upload_inputs(m, features, N, nullptr);
+    if (rc) return rc;
+    if ((rc = forward(m, nsrc, ntgt, nullptr))) return rc;
+    MHIPCHK(m, hipMemcpyAsync(out, m->out, sizeof(float) * N, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return PDOT_OK;
+}
+
+int pdot_mlp_loss_grad(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, const float* target,
+                       float* loss, float* grad) {
+    if (!m || !features || !target || nsrc < 0 || ntgt < 0 || nsrc + ntgt == 0) return PDOT_E_ARG;
+    const int64_t N = nsrc + ntgt;
+    int rc = upload_inputs(m, features, N, target);
+    if (rc) return rc;
+    if ((rc = forward(m, nsrc, ntgt, m->tgt))) return rc;
+    if ((rc = backward(m, nsrc, ntgt))) return rc;
+    if (loss) {
+        m->hbuf.resize(N);
+        MHIPCHK(m, hipMemcpyAsync(m->hbuf.data(), m->sq, sizeof(float) * N, hipMemcpyDeviceToHost, m->stream));
+    }
+    if (grad) MHIPCHK(m, hipMemcpyAsync(grad, m->g, sizeof(float) * m->np, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    if (loss) {
+        double s = 0.0;
+        for (int64_t i = 0; i < N; ++i) s += m->hbuf[i];
+        *loss = (float)(s / (double)N);
+    }
+    return PDOT_OK;
+}
+
+int pdot_mlp_adam_step(pdot_mlp* m, float lr, float beta1, float beta2, float eps) {
+    if (!m) return PDOT_E_ARG;
+    m->adam_t += 1;
+    const float c1 = 1.f - powf(beta1, (float)m->adam_t), c2 = 1.f - powf(beta2, (float)m->adam_t);
+    hipLaunchKernelGGL(adam_kernel, dim3(blocks_for(m->np)), dim3(256), 0, m->stream, m->p, m->g, m->m, m->v, m->np,
+                       lr, beta1, beta2, eps, c1, c2);
+    MHIPCHK(m, hipGetLastError());
+    return PDOT_OK;
+}
+
+int pdot_mlp_gemm_probe(pdot_mlp* m, const float* A, const float* B, float* C, int64_t M, int64_t N, int64_t K,
+                        int32_t trans_a, int32_t trans_b) {
+    if (!m || !A || !B || !C || M <= 0 || N <= 0 || K <= 0) return PDOT_E_ARG;
+    float *dA, *dB, *dC;
+    int rc;
+    if ((rc = malloc_f(m, &dA, M * K))) return rc;
+    if ((rc = malloc_f(m, &dB, K * N))) return rc;
+    if ((rc = malloc_f(m, &dC, M * N))) return rc;
+    MHIPCHK(m, hipMemcpyAsync(dA, A, sizeof(float) * M * K, hipMemcpyHostToDevice, m->stream));
+    MHIPCHK(m, hipMemcpyAsync(dB, B, sizeof(float) * K * N, hipMemcpyHostToDevice, m->stream));
+    const int64_t lda = trans_a ? M : K, ldb = trans_b ? K : N;
+    hipError_t e;
+    if (!trans_a && !trans_b) e = gemm<false, false>(m, dA, lda, dB, ldb, dC, N, M, N, K);
+    else if (trans_a && !trans_b) e = gemm<true, false>(m, dA, lda, dB, ldb, dC, N, M, N, K);
+    else if (!trans_a && trans_b) e = gemm<false, true>(m, dA, lda, dB, ldb, dC, N, M, N, K);
+    else e = gemm<true, true>(m, dA, lda, dB, ldb, dC, N, M, N, K);
+    MHIPCHK(m, e);
+    MHIPCHK(m, hipMemcpyAsync(C, dC, sizeof(float) * M * N, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    (void)hipFree(dA);
+    (void)hipFree(dB);
+    (void)hipFree(dC);
+    return PDOT_OK;
+}
+
+}  // extern "C"

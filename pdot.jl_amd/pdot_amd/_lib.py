@@ -86,6 +86,7 @@ class PointStats(ctypes.Structure):
 
 
 _c_double_p = ctypes.POINTER(ctypes.c_double)
+_fp = ctypes.POINTER(ctypes.c_float)
 _ctx_p = ctypes.c_void_p
 
 # name -> (restype, argtypes); the exported C-ABI surface
@@ -128,6 +129,18 @@ SIGNATURES = {
     "pdot_batch_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_batch_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_batch_device_bytes": (ctypes.c_int64, [_ctx_p]),
+    # learned warm start network
+
+    "pdot_mlp_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32]),
+    "pdot_mlp_destroy": (None, [ctypes.c_void_p]),
+    "pdot_mlp_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "pdot_mlp_num_params": (ctypes.c_int64, [ctypes.c_void_p]),
+    "pdot_mlp_set_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "pdot_mlp_get_params": (ctypes.c_int, [ctypes.c_void_p, _fp]),
+    "pdot_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, ctypes.c_int64, _fp]),
+    "pdot_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, ctypes.c_int64, _fp, _fp, _fp]),
+    "pdot_mlp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "pdot_mlp_gemm_probe": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, _fp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),
 }
 
 _lib = None

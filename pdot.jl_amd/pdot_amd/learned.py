@@ -1,0 +1,212 @@
+"""Learned warm start: a point-wise network with mean-pooled set context
+predicts the dual potentials (u, v) of a transport problem from its point
+clouds and masses; the prediction only sets the starting iterate of the
+same PDHG solve (X0 = 0, G0 = (C - u0) - v0).  Zero-start PDHG stays the
+fallback and the solver's own termination test is the judge: a warm-started
+solve that does not reach OPTIMAL within the budget of the zero start is
+rerun from zero.
+
+The network runs on the GPU through pdot_mlp (libpdot_hip.so): bf16 MFMA
+matrix products with FP32 accumulation, trained with Adam by the library's
+own forward and backward kernels.
+
+Features per point: (x1, x2, count * mass, side) with side +1 for sources
+and -1 for targets.  Targets are the solver's duals normalised for the
+(u + t, v - t) invariance of the dual objective: t = (a'u - b'v) / 2.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import time
+from dataclasses import dataclass
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+
+_fp = ctypes.POINTER(ctypes.c_float)
+
+MLP_SIGNATURES = {k: v for k, v in _lib.SIGNATURES.items() if k.startswith("pdot_mlp_")}
+
+HIDDEN = 128
+NUM_FEATURES = 4
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warm_start_weights.npz")
+
+
+def _f32(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(_fp)
+
+
+def features(x, y, a, b) -> np.ndarray:
+    """N x 4 float32 rows: sources (x, n a_i, +1), then targets (y, m b_j, -1)."""
+    x, y = np.asarray(x, dtype=np.float64), np.asarray(y, dtype=np.float64)
+    n, m = len(x), len(y)
+    fs = np.column_stack([x[:, 0], x[:, 1], n * np.asarray(a), np.ones(n)])
+    ft = np.column_stack([y[:, 0], y[:, 1], m * np.asarray(b), -np.ones(m)])
+    return np.ascontiguousarray(np.vstack([fs, ft]), dtype=np.float32)
+
+
+def normalize_duals(u, v, a, b):
+    """(u - t, v + t) with t = (a'u - b'v)/2: same dual objective, balanced."""
+    t = (np.dot(a, u) - np.dot(b, v)) / 2.0
+    return u - t, v + t
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """He-uniform weights, zero biases, in the pdot_mlp parameter order."""
+    rng = np.random.default_rng(seed)
+
+    def he(fan_in, shape):
+        lim = np.sqrt(6.0 / fan_in)
+        return rng.uniform(-lim, lim, size=shape)
+
+    W = HIDDEN
+    parts = [he(NUM_FEATURES, (NUM_FEATURES, W)), np.zeros(W), he(3 * W, (W, W)), he(3 * W, (W, W)),
+             he(3 * W, (W, W)), np.zeros(W), he(W, (W, W)), np.zeros(W), he(W, (W,)) * 0.1, np.zeros(1)]
+    return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
+
+
+class WarmStartNet:
+    """One pdot_mlp network on the GPU."""
+
+    def __init__(self, params: Optional[np.ndarray] = None, device: int = 0, seed: int = 0):
+        self.lib = _lib.load_library()
+        for name, (res, args) in MLP_SIGNATURES.items():
+            fn = getattr(self.lib, name)
+            fn.restype, fn.argtypes = res, args
+        h = ctypes.c_void_p()
+        rc = self.lib.pdot_mlp_create(ctypes.byref(h), int(device))
+        self.h = h
+        if rc != _lib.PDOT_OK:
+            msg = self.lib.pdot_mlp_last_error(h).decode() if h.value else str(rc)
+            raise _lib.PdotError(f"pdot_mlp_create failed: {msg}")
+        self.num_params = int(self.lib.pdot_mlp_num_params(h))
+        self.set_params(init_params(seed) if params is None else params)
+
+    def _check(self, rc, what):
+        if rc != _lib.PDOT_OK:
+            raise _lib.PdotError(f"{what} failed ({rc}): {self.lib.pdot_mlp_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h is not None and self.h.value:
+            self.lib.pdot_mlp_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, params) -> None:
+        p, pp = _f32(params)
+        if p.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters, got {p.size}")
+        self._check(self.lib.pdot_mlp_set_params(self.h, pp), "pdot_mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.empty(self.num_params, dtype=np.float32)
+        self._check(self.lib.pdot_mlp_get_params(self.h, p.ctypes.data_as(_fp)), "pdot_mlp_get_params")
+        return p
+
+    def forward(self, feats, ns: int, nt: int) -> np.ndarray:
+        f, fp = _f32(feats)
+        out = np.empty(ns + nt, dtype=np.float32)
+        self._check(self.lib.pdot_mlp_forward(self.h, fp, int(ns), int(nt), out.ctypes.data_as(_fp)), "pdot_mlp_forward")
+        return out
+
+    def loss_grad(self, feats, ns: int, nt: int, target, want_grad: bool = False):
+        f, fp = _f32(feats)
+        t, tp = _f32(target)
+        loss = ctypes.c_float()
+        g = np.empty(self.num_params, dtype=np.float32) if want_grad else None
+        self._check(self.lib.pdot_mlp_loss_grad(self.h, fp, int(ns), int(nt), tp, ctypes.byref(loss),
+                                                g.ctypes.data_as(_fp) if g is not None else None), "pdot_mlp_loss_grad")
+        return float(loss.value), g
+
+    def adam_step(self, lr: float, beta1: float = 0.9, beta2: float = 0.999, eps: float = 1e-8) -> None:
+        self._check(self.lib.pdot_mlp_adam_step(self.h, float(lr), float(beta1), float(beta2), float(eps)),
+                    "pdot_mlp_adam_step")
+
+    def gemm_probe(self, A, B, trans_a=False, trans_b=False) -> np.ndarray:
+        A, ap = _f32(A)
+        B, bp = _f32(B)
+        M = A.shape[1] if trans_a else A.shape[0]
+        K = A.shape[0] if trans_a else A.shape[1]
+        N = B.shape[0] if trans_b else B.shape[1]
+        C = np.empty((M, N), dtype=np.float32)
+        self._check(self.lib.pdot_mlp_gemm_probe(self.h, ap, bp, C.ctypes.data_as(_fp), M, N, K, int(trans_a),
+                                                 int(trans_b)), "pdot_mlp_gemm_probe")
+        return C
+
+    def predict_duals(self, x, y, a, b):
+        """Warm-start duals (u0, v0) in float64."""
+        n, m = len(x), len(y)
+        out = self.forward(features(x, y, a, b), n, m).astype(np.float64)
+        return out[:n], out[n:]
+
+
+@dataclass
+class Sample:
+    feats: np.ndarray
+    ns: int
+    nt: int
+    target: np.ndarray
+
+
+def make_sample(x, y, a, b, u, v) -> Sample:
+    un, vn = normalize_duals(u, v, a, b)
+    return Sample(features(x, y, a, b), len(x), len(y), np.concatenate([un, vn]).astype(np.float32))
+
+
+def train(net: WarmStartNet, samples: Sequence[Sample], epochs: int = 10, lr: float = 1e-3, seed: int = 0,
+          lr_decay: float = 0.97, log=None) -> List[float]:
+    """Adam over the samples (one instance per step), shuffled per epoch;
+    returns the mean training loss per epoch."""
+    rng = np.random.default_rng(seed)
+    history = []
+    for ep in range(epochs):
+        order = rng.permutation(len(samples))
+        tot = 0.0
+        for i in order:
+            s = samples[i]
+            loss, _ = net.loss_grad(s.feats, s.ns, s.nt, s.target)
+            net.adam_step(lr)
+            tot += loss
+        history.append(tot / len(samples))
+        if log:
+            log(f"epoch {ep}: loss {history[-1]:.4e} lr {lr:.2e}")
+        lr *= lr_decay
+    return history
+
+
+def evaluate_loss(net: WarmStartNet, samples: Sequence[Sample]) -> float:
+    return float(np.mean([net.loss_grad(s.feats, s.ns, s.nt, s.target)[0] for s in samples]))
+
+
+def save_params(path: str, params: np.ndarray, meta: dict) -> None:
+    np.savez(path, params=np.asarray(params, dtype=np.float32),
+             **{k: np.asarray(v) for k, v in meta.items()})
+
+
+def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+    with np.load(path, allow_pickle=False) as z:
+        return z["params"].astype(np.float32)
+
+
+def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True):
+    """optimize() from the network's duals; if that solve does not reach
+    OPTIMAL, the zero-start solve is run and returned (the fallback).
+    Returns (output, used_warm_start, predict_seconds)."""
+    from . import solver as S
+    from .solve_log import TerminationReason
+    t0 = time.time()
+    u0, v0 = net.predict_duals(x, y, problem.source_distribution, problem.target_distribution)
+    pred = time.time() - t0
+    out = S.optimize(params, problem, storage=storage, warm_start=(u0, v0))
+    if out.termination_reason == TerminationReason.TERMINATION_REASON_OPTIMAL or not fallback:
+        return out, True, pred
+    return S.optimize(params, problem, storage=storage), False, pred

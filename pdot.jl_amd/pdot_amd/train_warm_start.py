@@ -1,0 +1,137 @@
+"""Train the warm-start network and measure what it gains.
+
+    python -m pdot_amd.train_warm_start --out weights.npz --report report.json
+
+1. Training data: 2-D Gaussian-mixture instances (3-8 components, means
+   U[0,1]^2, sigma U[0.02, 0.15], Dirichlet(1) weights and masses, squared
+   Euclidean cost) of random sizes, solved by the batched plain solver
+   (zero start) to a tight tolerance; their duals, normalised for the
+   (u + t, v - t) invariance, are the targets.
+2. Training: Adam on the GPU through pdot_mlp, one instance per step.
+3. Held-out evaluation on instances of the same family at n = m = 4096 with
+   seeds disjoint from training: iterations and time to 1e-4 and 1e-6 from
+   the zero start and from the predicted duals (same solver, same
+   parameters; only the starting iterate differs).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import time
+
+import numpy as np
+
+from . import _lib, default_parameters, instances, learned
+from .batch import optimize_batch
+from .solve_log import TerminationReason
+from .solver import SolveProfile, optimize, relative_kkt_error
+
+TRAIN_SEED0 = 100000
+EVAL_SEED0 = 900000
+
+
+def generate_training_data(count, n_min, n_max, tol, seed, batch_size=512, iteration_limit=20000, log=print):
+    rng = np.random.default_rng(seed)
+    sizes = rng.integers(n_min, n_max + 1, size=(count, 2))
+    samples, kept, t0 = [], 0, time.time()
+    for start in range(0, count, batch_size):
+        idx = range(start, min(count, start + batch_size))
+        insts = [instances.gmm_dirichlet(int(sizes[k, 0]), int(sizes[k, 1]), seed=TRAIN_SEED0 + seed * 100000 + k)
+                 for k in idx]
+        outs = optimize_batch(default_parameters(tol, iteration_limit=iteration_limit, record_iteration_stats=False),
+                              [i.problem for i in insts], storage=_lib.STORAGE_F64)
+        for inst, out in zip(insts, outs):
+            if out.termination_reason != TerminationReason.TERMINATION_REASON_OPTIMAL:
+                continue
+            p = inst.problem
+            samples.append(learned.make_sample(inst.x, inst.y, p.source_distribution, p.target_distribution,
+                                               out.dual_source_solution, out.dual_target_solution))
+            kept += 1
+        log(f"data: {start + len(idx)}/{count} solved, {kept} kept, {time.time() - t0:.1f}s")
+    return samples
+
+
+def evaluate(net, count, n, tols=(1e-4, 1e-6), storage=_lib.STORAGE_F64, log=print):
+    rows = []
+    for k in range(count):
+        inst = instances.gmm_dirichlet(n, seed=EVAL_SEED0 + k)
+        p = inst.problem
+        row = {"seed": EVAL_SEED0 + k}
+        t0 = time.time()
+        u0, v0 = net.predict_duals(inst.x, inst.y, p.source_distribution, p.target_distribution)
+        row["predict_sec"] = time.time() - t0
+        dev = _lib.DeviceSolver(n, n, storage=storage)
+        dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
+        for tol in tols:
+            for name, ws in (("zero", None), ("warm", (u0, v0))):
+                prof = SolveProfile()
+                out = optimize(default_parameters(tol, record_iteration_stats=False), p, storage=storage,
+                               warm_start=ws, solver=dev, profile=prof)
+                ci = out.iteration_stats[-1].convergence_information[0]
+                row[f"{name}_{tol:g}"] = {"iterations": out.iteration_count, "seconds": prof.solve_sec,
+                                          "termination": out.termination_string,
+                                          "objective": ci.primal_objective, "rel_kkt": relative_kkt_error(ci)}
+        dev.close()
+        rows.append(row)
+        log(f"eval {k}: " + ", ".join(f"{key} {row[key]['iterations']}it {row[key]['seconds']:.2f}s"
+                                      for key in row if key.startswith(("zero", "warm"))))
+    summary = {}
+    for tol in tols:
+        z = np.array([r[f"zero_{tol:g}"]["iterations"] for r in rows], dtype=float)
+        w = np.array([r[f"warm_{tol:g}"]["iterations"] for r in rows], dtype=float)
+        zt = np.array([r[f"zero_{tol:g}"]["seconds"] for r in rows])
+        wt = np.array([r[f"warm_{tol:g}"]["seconds"] + r["predict_sec"] for r in rows])
+        summary[f"{tol:g}"] = {"iterations_zero_median": float(np.median(z)), "iterations_warm_median": float(np.median(w)),
+                               "iteration_ratio_median": float(np.median(z / w)),
+                               "time_zero_total": float(zt.sum()), "time_warm_total_incl_predict": float(wt.sum()),
+                               "time_ratio_total": float(zt.sum() / wt.sum()),
+                               "warm_optimal": int(sum(r[f"warm_{tol:g}"]["termination"] == "OPTIMAL" for r in rows))}
+    return rows, summary
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=1024, help="training instances")
+    ap.add_argument("--n-min", type=int, default=256)
+    ap.add_argument("--n-max", type=int, default=768)
+    ap.add_argument("--data-tol", type=float, default=1e-5)
+    ap.add_argument("--epochs", type=int, default=25)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--eval", type=int, default=8, help="held-out instances")
+    ap.add_argument("--eval-n", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--out", default=learned.DEFAULT_WEIGHTS)
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)  # noqa: E731
+
+    t0 = time.time()
+    samples = generate_training_data(args.train, args.n_min, args.n_max, args.data_tol, args.seed, log=log)
+    data_sec = time.time() - t0
+    rng = np.random.default_rng(args.seed)
+    order = rng.permutation(len(samples))
+    nval = max(1, len(samples) // 10)
+    val = [samples[i] for i in order[:nval]]
+    trn = [samples[i] for i in order[nval:]]
+    net = learned.WarmStartNet(seed=args.seed)
+    zero = float(np.mean([np.mean(s.target.astype(np.float64) ** 2) for s in val]))
+    t0 = time.time()
+    hist = learned.train(net, trn, epochs=args.epochs, lr=args.lr, seed=args.seed, log=log)
+    train_sec = time.time() - t0
+    val_loss = learned.evaluate_loss(net, val)
+    log(f"validation mse {val_loss:.4e} (predicting zero: {zero:.4e})")
+    params = net.get_params()
+    learned.save_params(args.out, params, {"train_instances": len(trn), "val_mse": val_loss, "epochs": args.epochs})
+    rows, summary = evaluate(net, args.eval, args.eval_n, log=log)
+    report = {"data_sec": data_sec, "train_sec": train_sec, "train_instances": len(trn), "val_instances": len(val),
+              "train_loss_history": hist, "val_mse": val_loss, "val_mse_zero_prediction": zero,
+              "eval_n": args.eval_n, "eval": rows, "summary": summary}
+    log(json.dumps(summary))
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(report, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
