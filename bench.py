@@ -173,6 +173,45 @@ def bench_batched(args) -> int:
     return 0
 
 
+def bench_warm_start(args) -> int:
+    """Held-out 4096 x 4096 Gaussian-mixture instances (FP32 storage): time to
+    1e-4 relative KKT from the zero start and through the learned warm start
+    (committed network weights, KKT judge and zero-start fallback), the
+    prediction time included."""
+    import numpy as np
+    import pdot_amd as P
+    from pdot_amd import _lib, instances, learned
+    net = learned.WarmStartNet(learned.load_params())
+    rows = []
+    for k in range(args.instances):
+        inst = instances.gmm_dirichlet(4096, seed=900200 + k)
+        p = inst.problem
+        prm = P.default_parameters(1e-4, record_iteration_stats=False)
+        t0 = time.time()
+        zero = P.optimize(prm, p, storage=_lib.STORAGE_F32)
+        tz = time.time() - t0
+        t0 = time.time()
+        warm, used, pred = learned.solve_warm(prm, p, inst.x, inst.y, net, storage=_lib.STORAGE_F32)
+        tw = time.time() - t0
+        rows.append({"seed": 900200 + k, "zero_iterations": zero.iteration_count, "zero_sec": tz,
+                     "warm_iterations": warm.iteration_count, "warm_sec": tw, "warm_used": bool(used),
+                     "predict_sec": pred, "warm_termination": warm.termination_string})
+        print(f"# instance {k}: zero {zero.iteration_count} it {tz:.2f}s, warm {warm.iteration_count} it {tw:.2f}s",
+              file=sys.stderr, flush=True)
+    net.close()
+    tz = sum(r["zero_sec"] for r in rows)
+    tw = sum(r["warm_sec"] for r in rows)
+    print(json.dumps({"metric": "learned warm start: time-to-1e-4 on held-out 4096x4096 GMM instances (FP32 storage)",
+                      "value": tz / tw, "unit": "speedup (zero-start time / warm-start time, incl. prediction)",
+                      "higher_is_better": True, "data": "synthetic", "n_gpus": 1,
+                      "config": {"workload": "gmm_dirichlet 4096x4096", "instances": len(rows), "storage": "f32"},
+                      "zero_sec_total": tz, "warm_sec_total": tw,
+                      "iteration_ratio_median": float(np.median([r["zero_iterations"] / max(1, r["warm_iterations"])
+                                                                 for r in rows])),
+                      "rows": rows}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,10 +221,11 @@ def main() -> int:
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--instances", type=int, default=4, help="held-out instances of the gmm_warm_start workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="gaussian_shift",
-                    choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform"),
+                    choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform", "gmm_warm_start"),
                     help="gaussian_shift: the headline 10k x 10k measurement; the others measure the 30k x 30k "
                          "FP32-storage stress case and the 512-instance batch")
     args = ap.parse_args()
@@ -197,6 +237,8 @@ def main() -> int:
         return bench_l1_mixture(args)
     if args.workload == "batched_uniform":
         return bench_batched(args)
+    if args.workload == "gmm_warm_start":
+        return bench_warm_start(args)
 
     from pdot_amd import _lib, instances
 

@@ -97,8 +97,17 @@ void pdot_destroy(pdot_ctx* ctx);
 const char* pdot_last_error(const pdot_ctx* ctx);
 
 /* Host-to-device copy of the problem, ot_cpu_to_gpu (src/io.jl:18-36).
- * C: n*m column-major FP64; a: n; b: m. */
+ * C: n*m column-major FP64; a: n; b: m.  C streams through a pinned
+ * two-buffer staging ring (host copy of one chunk overlapping the DMA of the
+ * previous one); FP32 storage is narrowed on the device; the same pass
+ * computes the norms of cached_ot_info. */
 int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b);
+
+/* cached_ot_info (src/termination.jl:94-101) of the uploaded problem, in the
+ * CachedOTInfo field order: ||C||_inf, ||[a;b]||_inf, ||C||_F, ||[a;b]||_2.
+ * The matrix norms come from the upload pass on the device (from the FP64
+ * input, for either storage); no extra host pass over C. */
+int pdot_cached_ot_info(pdot_ctx* ctx, double* out4);
 
 /* Reset the iterate: X = 0, r = c = 0, empty average, counters 0
  * (src/solver_gpu.jl:428-441, 476-496).  u0/v0 NULL gives the reference's zero
@@ -143,6 +152,30 @@ int pdot_set_state(pdot_ctx* ctx, const pdot_solver_state* in);
  * gpu_to_cpu! (src/io.jl:41-53).  X: n*m column-major FP64 (may be NULL),
  * u: n, v: m (may be NULL).  FP32 storage is widened on the device. */
 int pdot_download(pdot_ctx* ctx, int32_t which, double* X, double* u, double* v);
+
+/* Feasibility rounding of the current or average plan onto the transport
+ * polytope of (a, b), rounding(problem, primal_solution) of
+ * src/utils_gpu.jl:296-318, computed on the device from the resident point
+ * (FP32 storage is widened to FP64 first).  X_rounded: n*m column-major
+ * FP64.  The point itself is left unchanged. */
+int pdot_round(pdot_ctx* ctx, int32_t which, double* X_rounded);
+
+/* The same rounding of a host plan X (n*m column-major FP64), in place, as
+ * the reference's rounding() overwrites its primal_solution argument
+ * (src/utils_gpu.jl:300-318).  Needs no context; errbuf (errlen bytes, may be
+ * NULL) receives the error text. */
+int pdot_round_matrix(int32_t device, int64_t n, int64_t m, const double* a, const double* b, double* X,
+                      char* errbuf, int64_t errlen);
+
+/* Host-side text output, write_vector_to_file of scripts/solve.jl:5-11: one
+ * element per line (a matrix in column-major order), each as Julia's println
+ * prints a Float64 (shortest round-trip digits; "0.0001", "123456.0",
+ * "1.0e-5", "1.234567e6", "NaN", "Inf").  Formatting runs on host threads. */
+int pdot_write_julia_vector(const char* path, const double* x, int64_t count, char* errbuf, int64_t errlen);
+
+/* The Julia text of one Float64 into out (outlen >= 33); returns its length
+ * or a negative PDOT_E_* code. */
+int pdot_format_julia_float(double x, char* out, int64_t outlen);
 
 /* Timing hook: run `count` take_step! calls like pdot_run_steps while
  * recording HIP events on the context's stream around every launch of the

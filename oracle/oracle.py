@@ -95,7 +95,7 @@ def lib() -> ctypes.CDLL:
         L = ctypes.CDLL(LIB)
         L.oracle_optimize.restype = ctypes.c_int
         L.oracle_optimize.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp,
-                                      ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.c_int32, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_int32),
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(OracleStats), ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32), _dp, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int64)]
@@ -166,12 +166,14 @@ def optimize(params, problem, mode: int = FUSED, warm_start=None, stats_cap: int
     nst = ctypes.c_int32()
     trace = np.empty(max(1, trace_cap))
     tlen = ctypes.c_int64()
-    u0 = v0 = None
+    u0 = v0 = X0 = None
     if warm_start is not None:
         u0 = np.ascontiguousarray(warm_start[0], dtype=np.float64)
         v0 = np.ascontiguousarray(warm_start[1], dtype=np.float64)
+        if len(warm_start) > 2 and warm_start[2] is not None:
+            X0 = np.asfortranarray(warm_start[2], dtype=np.float64)
     op = params_from(params)
-    rc = L.oracle_optimize(ctypes.byref(op), n, m, _p(C), _p(a), _p(b), int(mode), _p(u0), _p(v0), _p(X), _p(u),
+    rc = L.oracle_optimize(ctypes.byref(op), n, m, _p(C), _p(a), _p(b), int(mode), _p(u0), _p(v0), _p(X0), _p(X), _p(u),
                            _p(v), ctypes.byref(reason), ctypes.byref(iters), stats, stats_cap, ctypes.byref(nst),
                            _p(trace), trace_cap, ctypes.byref(tlen))
     if rc != 0:

@@ -553,7 +553,7 @@ static grad_t point_grad(const point_t* pt) {
 }
 
 int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* C, const double* a,
-                    const double* b, int32_t mode, const double* u0, const double* v0, double* X_out,
+                    const double* b, int32_t mode, const double* u0, const double* v0, const double* X0, double* X_out,
                     double* u_out, double* v_out, int32_t* termination_reason, int32_t* iteration_count,
                     oracle_stats* stats, int32_t stats_cap, int32_t* n_stats, double* eta_trace,
                     int64_t trace_cap, int64_t* trace_len) {
@@ -604,7 +604,13 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
     last.u = vp; vp += n; last.v = vp; vp += m; last.r = vp; vp += n; last.c = vp; vp += m;
 
     /* initial state: X = 0, duals 0 (or a warm start), G = 0 (solver_gpu.jl:429-434);
-     * a warm start defines G0 = (C - u0) - v0. */
+     * a warm start defines G0 = (C - u0) - v0 and may also set X0 (then r, c
+     * are its row and column sums). */
+    if (X0) {
+        memcpy(cur.X, X0, sizeof(double) * (size_t)nm);
+        row_sums(n, m, cur.X, NULL, cur.r);
+        col_sums(n, m, cur.X, NULL, cur.c);
+    }
     cur.gzero = 1;
     if (u0 && v0) {
         memcpy(cur.u, u0, sizeof(double) * n);

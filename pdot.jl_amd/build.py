@@ -10,7 +10,7 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(HERE)
-SRCS = [os.path.join(HERE, "csrc", f) for f in ("pdot_hip.hip", "pdot_batch.hip", "pdot_mlp.hip")]
+SRCS = [os.path.join(HERE, "csrc", f) for f in ("pdot_hip.hip", "pdot_batch.hip", "pdot_mlp.hip", "pdot_round.hip", "pdot_io.cpp")]
 OUT = os.path.join(HERE, "pdot_amd", "libpdot_hip.so")
 
 HIPCC_FLAGS = [

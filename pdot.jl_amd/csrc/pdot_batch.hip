@@ -36,6 +36,10 @@
 namespace {
 
 constexpr int kBWaves = 4;                 // batched step kernel: waves per block
+#ifndef PDOT_BATCH_PIPE
+#define PDOT_BATCH_PIPE 0
+#endif
+constexpr int kBPipe = PDOT_BATCH_PIPE;  // column batches prefetched ahead in a tile
 constexpr int kBBlock = kWave * kBWaves;
 constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block
 constexpr int kBRedBlock = kWave * kBRedWaves;
@@ -49,7 +53,8 @@ struct BInst {
     int64_t u[4], r[4];       // double offsets into the vector arena (length ld)
     int64_t v[4], c[4];       // (length m)
     int64_t a, b;
-    int64_t rowpart, colpart, sqpart, evalpart, redpart, evalvec;  // offsets into the partial arena
+    int64_t rowpart, colpart, sqpart, evalpart;  // offsets into the partial arena
+    int64_t redpart, evalvec;                    // offsets into the reduction arena
     int32_t R, Cb, TC, nstep, NBr, NBc, nred, sqchunk;
 };
 
@@ -63,7 +68,8 @@ struct BChunk {
 struct BParams {
     void* mat;
     double* vec;
-    double* part;
+    double* part;  // per-pass partial sums (layout follows the active set)
+    double* red;   // per-instance pass reductions and eval vector partials (fixed layout)
     const BInst* inst;
     pdot_solver_state* st;
     unsigned int* counter;
@@ -136,7 +142,7 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             a.gzero = sst.gzero != 0;
             a.do_step = do_step;
             double dd = 0.0, ev[kMatStats];
-            step_tile<T, EVAL, 2, kBWaves>(a, colbuf, red, &dd, ev);
+            step_tile<T, EVAL, 2, kBWaves, kBPipe>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
                 const int64_t lb = stl.rb + (int64_t)stl.cb * sinst.R;
                 if (do_step) p.part[sinst.sqpart + lb] = dd;
@@ -204,10 +210,10 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
             reduce_chunk<EVAL, kBRedWaves>(c, psum, out3, ev);
             if (tid == 0) {
                 if (EVAL) {
-                    for (int q = 0; q < kVecStats; ++q) p.part[sinst.evalvec + lc * kVecStats + q] = ev[q];
+                    for (int q = 0; q < kVecStats; ++q) p.red[sinst.evalvec + lc * kVecStats + q] = ev[q];
                 }
                 if (do_step) {
-                    double* rp = p.part + sinst.redpart + 3 * lc;
+                    double* rp = p.red + sinst.redpart + 3 * lc;
                     rp[0] = out3[0];
                     rp[1] = out3[1];
                     rp[2] = out3[2];
@@ -220,7 +226,7 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
             if (amlast) {
                 __threadfence();
                 if (tid < kWave) {
-                    if (!EVAL) reduce_and_commit(&sst, p.part + sinst.redpart, sinst.nred);
+                    if (!EVAL) reduce_and_commit(&sst, p.red + sinst.redpart, sinst.nred);
                     if (tid == 0) {
                         if (EVAL) sst.pending = 1;
                         p.st[sch.inst] = sst;
@@ -242,7 +248,7 @@ __global__ __launch_bounds__(kWave) void bcommit_kernel(BParams p, int32_t count
     __syncthreads();
     if (!sst.pending) return;
     const BInst& in = p.inst[k];
-    reduce_and_commit(&sst, p.part + in.redpart, in.nred);
+    reduce_and_commit(&sst, p.red + in.redpart, in.nred);
     if (threadIdx.x == 0) p.st[k] = sst;
 }
 
@@ -268,7 +274,7 @@ __global__ __launch_bounds__(kBBlock) void bevalfin_kernel(BParams p, const int3
         const bool is_max = (q == 0 || q == 3 || q == 6 || q == 9);
         double acc = 0.0;
         for (int64_t b = tid; b < in.nred; b += kBBlock) {
-            const double v = p.part[in.evalvec + b * kVecStats + q];
+            const double v = p.red[in.evalvec + b * kVecStats + q];
             acc = is_max ? nanmax(acc, v) : acc + v;
         }
         acc = is_max ? block_max(acc, red) : block_sum(acc, red);
@@ -307,6 +313,8 @@ struct pdot_batch {
     void* mat = nullptr;
     double* vec = nullptr;
     double* part = nullptr;
+    int64_t part_cap = 0;  // doubles
+    double* red = nullptr;
     BInst* d_inst = nullptr;
     pdot_solver_state* d_st = nullptr;
     unsigned int* d_counter = nullptr;
@@ -320,7 +328,7 @@ struct pdot_batch {
     pdot_solver_state* h_st = nullptr;  // pinned, count entries
     double* h_eval = nullptr;           // pinned, count * kEvalOut
     std::vector<int32_t> active;
-    int64_t max_tiles = 0, max_chunks = 0;
+    int64_t max_tiles = 0, max_chunks = 0, tile_cap = 0;
     int32_t ntiles = 0, nchunks = 0;
     int32_t cols_per_tile = 64;
     int32_t step_grid = 0, reduce_grid = 0, grid_mult = 1;
@@ -364,6 +372,7 @@ BParams bparams(pdot_batch* b) {
     p.mat = b->mat;
     p.vec = b->vec;
     p.part = b->part;
+    p.red = b->red;
     p.inst = b->d_inst;
     p.st = b->d_st;
     p.counter = b->d_counter;
@@ -379,34 +388,54 @@ void bfree_graphs(pdot_batch* b) {
     b->graphs.clear();
 }
 
-// Partial-buffer layout per instance for the current tile width; returns the
-// arena size in doubles.
-int64_t layout_partials(pdot_batch* b) {
-    const int TR = rows_per_tile(b->storage);
+// Reduction-arena layout (independent of the tile shape): the pass
+// reductions a pending pass is committed from, and the evaluation vector
+// partials.  Returns the arena size in doubles.
+int64_t layout_reductions(pdot_batch* b) {
     int64_t off = 0;
-    b->max_tiles = 0;
-    b->max_chunks = 0;
     for (auto& in : b->inst) {
-        in.R = (int32_t)((in.n + TR - 1) / TR);
-        // at least kMinColTiles column tiles per instance, so that a small
-        // instance left alone in the batch is spread over many blocks instead
-        // of one block walking its columns one load latency at a time
-        in.TC = (int32_t)std::min<int64_t>(b->cols_per_tile,
-                                           std::max<int64_t>(kMinTileCols, (in.m + kMinColTiles - 1) / kMinColTiles));
-        in.Cb = (int32_t)((in.m + in.TC - 1) / in.TC);
-        in.nstep = in.R * in.Cb;
         in.NBr = (int32_t)((in.n + kWave - 1) / kWave);
         in.NBc = (int32_t)((in.m + kWave - 1) / kWave);
         in.nred = in.NBr + in.NBc;
+        in.redpart = off; off += 3 * (int64_t)in.nred;
+        in.evalvec = off; off += (int64_t)in.nred * kVecStats;
+    }
+    return off;
+}
+
+// Tile shape and partial-arena layout for the active set; returns the arena
+// size in doubles.  Every active instance gets TC columns per tile with
+//   TC = clamp(min(ceil(W / G), ceil(m / kMinColTiles)), kMinTileCols, cols_per_tile)
+// where W = sum over active instances of (row tiles x columns) and G the
+// resident step blocks: while many instances are active the tuned width
+// applies; as they finish, the tiles narrow so that the remaining instances
+// still spread over the whole grid instead of a few blocks walking their
+// columns one load latency at a time.
+int64_t layout_partials(pdot_batch* b) {
+    const int TR = rows_per_tile(b->storage);
+    const int64_t slots = std::max<int64_t>(1, (int64_t)b->step_resident * b->grid_mult);
+    int64_t work = 0;
+    for (int32_t k = 0; k < b->count; ++k) {
+        BInst& in = b->inst[k];
+        in.R = (int32_t)((in.n + TR - 1) / TR);
+        if (b->active[k]) work += (int64_t)in.R * in.m;
+    }
+    const int64_t tct = std::max<int64_t>(1, (work + slots - 1) / slots);
+    int64_t off = 0;
+    b->max_tiles = 0;
+    for (int32_t k = 0; k < b->count; ++k) {
+        BInst& in = b->inst[k];
+        const int64_t want = std::min<int64_t>(tct, (in.m + kMinColTiles - 1) / kMinColTiles);
+        in.TC = (int32_t)std::min<int64_t>(b->cols_per_tile, std::max<int64_t>(kMinTileCols, want));
+        in.Cb = (int32_t)((in.m + in.TC - 1) / in.TC);
+        in.nstep = in.R * in.Cb;
         in.sqchunk = (in.nstep + in.nred - 1) / in.nred;
+        if (!b->active[k]) continue;  // no kernel touches an inactive instance's partials
         in.rowpart = off; off += (int64_t)in.Cb * in.ld;
         in.colpart = off; off += (int64_t)in.R * in.m;
         in.sqpart = off; off += in.nstep;
         in.evalpart = off; off += (int64_t)in.nstep * kMatStats;
-        in.redpart = off; off += 3 * (int64_t)in.nred;
-        in.evalvec = off; off += (int64_t)in.nred * kVecStats;
         b->max_tiles += in.nstep;
-        b->max_chunks += in.nred;
     }
     return off;
 }
@@ -448,22 +477,41 @@ int rebuild_work(pdot_batch* b) {
 }
 
 int setup_batch_partials(pdot_batch* b) {
-    bfree_graphs(b);
-    if (b->part) (void)hipFree(b->part);
-    if (b->d_tiles) (void)hipFree(b->d_tiles);
-    if (b->d_chunks) (void)hipFree(b->d_chunks);
-    b->part = nullptr;
-    b->d_tiles = nullptr;
-    b->d_chunks = nullptr;
     const int64_t nd = layout_partials(b);
     int rc;
-    if ((rc = balloc(b, (void**)&b->part, sizeof(double) * (size_t)nd))) return rc;
-    if ((rc = balloc(b, (void**)&b->d_tiles, sizeof(BTile) * (size_t)b->max_tiles))) return rc;
-    if ((rc = balloc(b, (void**)&b->d_chunks, sizeof(BChunk) * (size_t)b->max_chunks))) return rc;
+    if (nd > b->part_cap || !b->part) {
+        bfree_graphs(b);
+        if (b->part) {
+            (void)hipFree(b->part);
+            b->bytes -= (int64_t)sizeof(double) * b->part_cap;
+        }
+        b->part = nullptr;
+        b->part_cap = 0;
+        if ((rc = balloc(b, (void**)&b->part, sizeof(double) * (size_t)nd))) return rc;
+        b->part_cap = nd;
+    }
+    if (b->max_tiles > b->tile_cap || !b->d_tiles) {
+        bfree_graphs(b);
+        if (b->d_tiles) {
+            (void)hipFree(b->d_tiles);
+            b->bytes -= (int64_t)sizeof(BTile) * b->tile_cap;
+        }
+        b->d_tiles = nullptr;
+        b->tile_cap = 0;
+        if ((rc = balloc(b, (void**)&b->d_tiles, sizeof(BTile) * (size_t)b->max_tiles))) return rc;
+        b->tile_cap = b->max_tiles;
+    }
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
     BHIPCHK(b, hipMemcpy(b->d_inst, b->inst.data(), sizeof(BInst) * b->count, hipMemcpyHostToDevice));
-    b->step_grid = std::max(1, b->step_resident * b->grid_mult);
-    b->reduce_grid = std::max(1, b->reduce_resident);
-    return rebuild_work(b);
+    if ((rc = rebuild_work(b))) return rc;
+    // persistent grids sized to the work: no idle blocks are launched once
+    // few instances remain
+    const int32_t sg = (int32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)b->step_resident * b->grid_mult, b->ntiles));
+    const int32_t rg = std::max(1, std::min(b->reduce_resident, b->nchunks));
+    if (sg != b->step_grid || rg != b->reduce_grid) bfree_graphs(b);  // captured launches hold the grid
+    b->step_grid = sg;
+    b->reduce_grid = rg;
+    return PDOT_OK;
 }
 
 hipError_t blaunch_step(pdot_batch* b, bool eval) {
@@ -585,6 +633,11 @@ int pdot_batch_create(pdot_batch** out, int32_t count, const int64_t* n, const i
     if ((rc = balloc(b, (void**)&b->d_nwork, sizeof(int32_t) * 2))) return rc;
     if ((rc = balloc(b, (void**)&b->d_active, sizeof(int32_t) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_evalout, sizeof(double) * kEvalOut * count))) return rc;
+    const int64_t nred_total = layout_reductions(b);
+    if ((rc = balloc(b, (void**)&b->red, sizeof(double) * (size_t)nred_total))) return rc;
+    b->max_chunks = 0;
+    for (const auto& in : b->inst) b->max_chunks += in.nred;
+    if ((rc = balloc(b, (void**)&b->d_chunks, sizeof(BChunk) * (size_t)b->max_chunks))) return rc;
     BHIPCHK(b, hipMemsetAsync(b->d_counter, 0, sizeof(unsigned int) * count, b->stream));
     BHIPCHK(b, hipMemsetAsync(b->d_st, 0, sizeof(pdot_solver_state) * count, b->stream));
     BHIPCHK(b, hipHostMalloc((void**)&b->h_st, sizeof(pdot_solver_state) * count, hipHostMallocDefault));
@@ -609,7 +662,7 @@ void pdot_batch_destroy(pdot_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     bfree_graphs(b);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
-    void* ps[] = {b->mat, b->vec, b->part, b->d_inst, b->d_st, b->d_counter, b->d_tiles, b->d_chunks,
+    void* ps[] = {b->mat, b->vec, b->part, b->red, b->d_inst, b->d_st, b->d_counter, b->d_tiles, b->d_chunks,
                   b->d_nwork, b->d_active, b->d_evalout, b->stage};
     for (void* q : ps)
         if (q) (void)hipFree(q);
@@ -675,7 +728,7 @@ int pdot_batch_set_start(pdot_batch* b, int32_t step_policy, double reduction_ex
     int rc = bwrite_states(b);
     if (rc) return rc;
     b->active.assign(b->count, 1);
-    if ((rc = rebuild_work(b))) return rc;
+    if ((rc = setup_batch_partials(b))) return rc;
     b->started = true;
     return PDOT_OK;
 }
@@ -697,8 +750,13 @@ int pdot_batch_set_warm_start(pdot_batch* b, int32_t k, const double* u0, const 
 
 int pdot_batch_set_active(pdot_batch* b, const int32_t* active) {
     if (!b || !active) return PDOT_E_ARG;
-    for (int32_t k = 0; k < b->count; ++k) b->active[k] = active[k] ? 1 : 0;
-    return rebuild_work(b);
+    bool same = true;
+    for (int32_t k = 0; k < b->count; ++k) {
+        const int32_t a = active[k] ? 1 : 0;
+        same = same && a == b->active[k];
+        b->active[k] = a;
+    }
+    return same ? PDOT_OK : setup_batch_partials(b);
 }
 
 int pdot_batch_get_states(pdot_batch* b, pdot_solver_state* out) {

@@ -16,11 +16,7 @@
 namespace {
 
 constexpr int kWave = 64;
-#ifndef PDOT_STEP_WAVES
-#define PDOT_STEP_WAVES 8
-#endif
-constexpr int kWaves = PDOT_STEP_WAVES;  // step kernel: waves per block, stacked along rows
-constexpr int kBlock = kWave * kWaves;
+constexpr int kBlock = 512;  // small single-block finishing kernels
 #ifndef PDOT_RED_WAVES
 #define PDOT_RED_WAVES 8
 #endif
@@ -31,10 +27,37 @@ constexpr int kLdAlign = 64;      // leading-dimension alignment (elements)
 constexpr int kMatStats = 10;     // per-block matrix eval reductions
 constexpr int kVecStats = 12;     // per-block vector eval reductions
 constexpr int kEvalOut = kMatStats + kVecStats;
-#ifndef PDOT_STEP_UNROLL
-#define PDOT_STEP_UNROLL 2
+// Shape of the single-problem step kernel per storage type (measured best on
+// MI355X at 10k x 10k): waves per block (stacked along rows), columns per
+// load batch, and how many batches are prefetched ahead of the one computed.
+#ifndef PDOT_F64_WAVES
+#define PDOT_F64_WAVES 8
 #endif
-constexpr int kUnroll = PDOT_STEP_UNROLL;  // columns whose loads are issued together
+#ifndef PDOT_F64_UNROLL
+#define PDOT_F64_UNROLL 1
+#endif
+#ifndef PDOT_F64_PIPE
+#define PDOT_F64_PIPE 1
+#endif
+#ifndef PDOT_F32_WAVES
+#define PDOT_F32_WAVES 4
+#endif
+#ifndef PDOT_F32_UNROLL
+#define PDOT_F32_UNROLL 2
+#endif
+#ifndef PDOT_F32_PIPE
+#define PDOT_F32_PIPE 1
+#endif
+template <typename T>
+struct StepShape;
+template <>
+struct StepShape<double> {
+    static constexpr int kWaves = PDOT_F64_WAVES, kUnroll = PDOT_F64_UNROLL, kPipe = PDOT_F64_PIPE;
+};
+template <>
+struct StepShape<float> {
+    static constexpr int kWaves = PDOT_F32_WAVES, kUnroll = PDOT_F32_UNROLL, kPipe = PDOT_F32_PIPE;
+};
 
 // ------------------------------------------------------------------ helpers
 
@@ -321,7 +344,7 @@ struct TileArgs {
 // evaluation reductions of the input point and of its average.  Called by
 // all threads of the block; thread 0 receives ||Delta||^2 of the tile in
 // *dd_out and the kMatStats evaluation reductions in ev_out.
-template <typename T, bool EVAL, int UNROLL, int WAVES>
+template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE>
 __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)[kChunk], double* red, double* dd_out,
                                           double* ev_out) {
     constexpr int VEC = VecT<T>::N;
@@ -343,6 +366,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const double* __restrict__ vc = a.vc;
     const double* __restrict__ va = a.va;
     const double eta = a.eta, omega = a.omega, nw = a.nw, aw = a.aw;
+    const double tau = eta / omega;
     const bool gzero = a.gzero;
     const int64_t j0 = a.j0, jstep = a.jstep, ncols = a.ncols;
 
@@ -362,94 +386,120 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
 #pragma unroll
     for (int k = 0; k < kMatStats; ++k) ev[k] = 0.0;
 
-    for (int64_t tc = 0; tc < ncols; tc += kChunk) {
-        const int64_t tend = (tc + kChunk < ncols) ? tc + kChunk : ncols;
-        for (int64_t tb = tc; tb < tend; tb += UNROLL) {
-            V xv[UNROLL], cv[UNROLL], av[UNROLL];
-            double vj[UNROLL], vaj[UNROLL];
+    // A batch is UNROLL consecutive tile columns whose loads are issued
+    // together; kChunk is a multiple of UNROLL, so a batch never straddles a
+    // column-sum flush.  With PIPE > 0 the loads of the next PIPE batches are
+    // issued before a batch is computed (a ring of PIPE + 1 register sets), so
+    // every wave keeps loads in flight while it computes and stores.
+    struct Batch {
+        V xv[UNROLL], cv[UNROLL], av[UNROLL];
+        double vj[UNROLL], vaj[UNROLL];
+    };
+    auto load_batch = [&](int64_t tb, Batch& B) {
 #pragma unroll
-            for (int q = 0; q < UNROLL; ++q) {
-                const bool colok = tb + q < tend;
-                const int64_t j = j0 + (tb + q) * jstep;
-                vj[q] = colok ? vc[j] : 0.0;
-                vaj[q] = (EVAL && colok) ? va[j] : 0.0;
-                if (colok && lane_active) {
-                    const int64_t idx = j * ld + i0;
-                    xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
-                    cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
-                    if (!avg_empty) av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
-                }
+        for (int q = 0; q < UNROLL; ++q) {
+            const bool colok = tb + q < ncols;
+            const int64_t j = j0 + (tb + q) * jstep;
+            B.vj[q] = colok ? vc[j] : 0.0;
+            B.vaj[q] = (EVAL && colok) ? va[j] : 0.0;
+            if (colok && lane_active) {
+                const int64_t idx = j * ld + i0;
+                B.xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
+                B.cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
+                if (!avg_empty) B.av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
             }
+        }
+    };
+    auto compute_batch = [&](int64_t tb, const Batch& B) {
+        const int64_t tc = tb / kChunk * kChunk;
 #pragma unroll
-            for (int q = 0; q < UNROLL; ++q) {
-                const bool colok = tb + q < tend;
-                const int64_t j = j0 + (tb + q) * jstep;
-                double csl = 0.0;
-                if (colok && lane_active) {
-                    const int64_t idx = j * ld + i0;
-                    double x[VEC], c[VEC], xa[VEC], xn[VEC], xan[VEC];
-                    unpack<V, T>(xv[q], x);
-                    unpack<V, T>(cv[q], c);
-                    if (!avg_empty) unpack<V, T>(av[q], xa);
+        for (int q = 0; q < UNROLL; ++q) {
+            const bool colok = tb + q < ncols;
+            const int64_t j = j0 + (tb + q) * jstep;
+            double csl = 0.0;
+            if (colok && lane_active) {
+                const int64_t idx = j * ld + i0;
+                double x[VEC], c[VEC], xa[VEC], xn[VEC], xan[VEC];
+                unpack<V, T>(B.xv[q], x);
+                unpack<V, T>(B.cv[q], c);
+                if (!avg_empty) unpack<V, T>(B.av[q], xa);
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) {
-                        // G_ij = (C_ij - u_i) - v_j (solver_gpu.jl:249-250); 0 at start (:434)
-                        double g = (gzero || !valid[e]) ? 0.0 : (c[e] - ui[e]) - vj[q];
-                        if (EVAL) {
-                            ev[0] += c[e] * x[e];
-                            ev[1] = nanmax(ev[1], fabs(x[e]));
-                            ev[2] += x[e] * x[e];
-                            const double rc = jl_min0(g);
-                            ev[3] = nanmax(ev[3], fabs(rc));
-                            ev[4] += rc * rc;
-                            if (!avg_empty) {
-                                const double ga = valid[e] ? (c[e] - uai[e]) - vaj[q] : 0.0;
-                                ev[5] += c[e] * xa[e];
-                                ev[6] = nanmax(ev[6], fabs(xa[e]));
-                                ev[7] += xa[e] * xa[e];
-                                const double rca = jl_min0(ga);
-                                ev[8] = nanmax(ev[8], fabs(rca));
-                                ev[9] += rca * rca;
-                            }
-                        }
-                        // Delta = max(X - (eta/omega) G, 0) - X  (solver_gpu.jl:139-141)
-                        double d = x[e] - (eta / omega) * g;
-                        d = jl_max0(d);
-                        double xnew;
-                        if constexpr (sizeof(T) == 8) {
-                            d -= x[e];
-                            xnew = x[e] + d;  // X .+= Delta (solver_gpu.jl:242)
-                        } else {
-                            xnew = (double)(float)d;  // FP32 storage: Delta of the stored values
-                            d = xnew - x[e];
-                        }
-                        xn[e] = xnew;
-                        rs[e] += d;
-                        csl += d;
-                        dd += d * d;
-                        // avg = (avg + x*nw)*aw (saddle_point_gpu.jl:129-130)
+                for (int e = 0; e < VEC; ++e) {
+                    // G_ij = (C_ij - u_i) - v_j (solver_gpu.jl:249-250); 0 at start (:434)
+                    double g = (gzero || !valid[e]) ? 0.0 : (c[e] - ui[e]) - B.vj[q];
+                    if (EVAL) {
+                        ev[0] += c[e] * x[e];
+                        ev[1] = nanmax(ev[1], fabs(x[e]));
+                        ev[2] += x[e] * x[e];
+                        const double rc = jl_min0(g);
+                        ev[3] = nanmax(ev[3], fabs(rc));
+                        ev[4] += rc * rc;
                         if (!avg_empty) {
-                            double t = xa[e] + xnew * nw;
-                            xan[e] = t * aw;
+                            const double ga = valid[e] ? (c[e] - uai[e]) - B.vaj[q] : 0.0;
+                            ev[5] += c[e] * xa[e];
+                            ev[6] = nanmax(ev[6], fabs(xa[e]));
+                            ev[7] += xa[e] * xa[e];
+                            const double rca = jl_min0(ga);
+                            ev[8] = nanmax(ev[8], fabs(rca));
+                            ev[9] += rca * rca;
                         }
                     }
-                    if (do_step) {
-                        store_vec<T>(Xn + idx, xn);
-                        if (!avg_empty) store_vec<T>(Xan + idx, xan);
+                    // Delta = max(X - (eta/omega) G, 0) - X  (solver_gpu.jl:139-141)
+                    double d = x[e] - tau * g;
+                    d = jl_max0(d);
+                    double xnew;
+                    if constexpr (sizeof(T) == 8) {
+                        d -= x[e];
+                        xnew = x[e] + d;  // X .+= Delta (solver_gpu.jl:242)
+                    } else {
+                        xnew = (double)(float)d;  // FP32 storage: Delta of the stored values
+                        d = xnew - x[e];
+                    }
+                    xn[e] = xnew;
+                    rs[e] += d;
+                    csl += d;
+                    dd += d * d;
+                    // avg = (avg + x*nw)*aw (saddle_point_gpu.jl:129-130)
+                    if (!avg_empty) {
+                        double t = xa[e] + xnew * nw;
+                        xan[e] = t * aw;
                     }
                 }
-                const double s = wave_sum(csl);
-                if (colok && lane == 0) colbuf[w][tb + q - tc] = s;
+                if (do_step) {
+                    store_vec<T>(Xn + idx, xn);
+                    if (!avg_empty) store_vec<T>(Xan + idx, xan);
+                }
             }
+            const double s = wave_sum(csl);
+            if (colok && lane == 0) colbuf[w][tb + q - tc] = s;
         }
-        __syncthreads();
-        if (do_step && tid < tend - tc) {
-            double t = colbuf[0][tid];
+        // flush the column sums after the chunk's last batch (block-uniform)
+        const int64_t tend = (tc + kChunk < ncols) ? tc + kChunk : ncols;
+        if (tb + UNROLL >= tend) {
+            __syncthreads();
+            if (do_step && tid < tend - tc) {
+                double t = colbuf[0][tid];
 #pragma unroll
-            for (int ww = 1; ww < WAVES; ++ww) t += colbuf[ww][tid];
-            a.colpart[j0 + (tc + tid) * jstep] = t;
+                for (int ww = 1; ww < WAVES; ++ww) t += colbuf[ww][tid];
+                a.colpart[j0 + (tc + tid) * jstep] = t;
+            }
+            __syncthreads();
         }
-        __syncthreads();
+    };
+    static_assert(kChunk % UNROLL == 0, "a batch must not straddle a column-sum flush");
+    Batch ring[PIPE + 1];
+#pragma unroll
+    for (int k = 0; k < PIPE; ++k)
+        if ((int64_t)k * UNROLL < ncols) load_batch((int64_t)k * UNROLL, ring[k]);
+    for (int64_t tb = 0; tb < ncols; tb += (int64_t)(PIPE + 1) * UNROLL) {
+#pragma unroll
+        for (int s = 0; s <= PIPE; ++s) {
+            const int64_t t = tb + (int64_t)s * UNROLL;
+            if (t >= ncols) break;  // block-uniform
+            const int64_t ahead = t + (int64_t)PIPE * UNROLL;
+            if (ahead < ncols) load_batch(ahead, ring[(s + PIPE) % (PIPE + 1)]);
+            compute_batch(t, ring[s]);
+        }
     }
 
     if (do_step && lane_active) {

@@ -38,10 +38,12 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "pdot.h"
 #include "pdot_common.h"
+#include "pdot_round.h"
 
 namespace {
 
@@ -86,12 +88,13 @@ struct ReduceParams {
 // One PDHG pass over the whole problem: block (rb, cb) owns rows
 // [rb*TR, (rb+1)*TR) of the columns cb, cb+Cb, ... (interleaved) or of the
 // contiguous range [cb*TC, (cb+1)*TC); step_tile does the work.
-template <typename T, bool EVAL, int UNROLL>
-__global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
-    constexpr int TR = kWaves * kWave * VecT<T>::N;
+template <typename T, bool EVAL>
+__global__ __launch_bounds__(StepShape<T>::kWaves * kWave) void step_kernel(StepParams p) {
+    constexpr int WAVES = StepShape<T>::kWaves;
+    constexpr int TR = WAVES * kWave * VecT<T>::N;
     __shared__ pdot_solver_state sst;
-    __shared__ double red[kBlock];
-    __shared__ double colbuf[kWaves][kChunk];
+    __shared__ double red[WAVES * kWave];
+    __shared__ double colbuf[WAVES][kChunk];
     const int tid = threadIdx.x;
     if (tid == 0) sst = *p.st;
     __syncthreads();
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void step_kernel(StepParams p) {
     a.gzero = sst.gzero != 0;
     a.do_step = do_step;
     double dd = 0.0, ev[kMatStats];
-    step_tile<T, EVAL, UNROLL, kWaves>(a, colbuf, red, &dd, ev);
+    step_tile<T, EVAL, StepShape<T>::kUnroll, WAVES, StepShape<T>::kPipe>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
         if (do_step) p.sqpart[blockIdx.x] = dd;
         if (EVAL) {
@@ -268,6 +271,35 @@ __global__ __launch_bounds__(kWave) void commit_kernel(pdot_solver_state* st, co
     if (threadIdx.x == 0) *st = sst;
 }
 
+constexpr int kIngestBlocks = 1024;  // blocks per ingest chunk
+
+// Ingest of one column block of the FP64 cost matrix (n x ncols, source
+// pitch lds): per-block max |C| (NaN propagates, as Julia's maximum does) and
+// sum of C^2, the inputs of cached_ot_info (src/termination.jl:94-101); with
+// NARROW the block is also stored as FP32 (pitch ld).  Block b takes columns
+// b, b + gridDim.x, ...; threads stride the rows.
+template <bool NARROW>
+__global__ __launch_bounds__(256) void ingest_kernel(const double* __restrict__ src, int64_t lds, float* __restrict__ dst,
+                                                     int64_t ld, int64_t n, int64_t ncols, double* __restrict__ part) {
+    __shared__ double red[256];
+    double mx = 0.0, ss = 0.0;
+    for (int64_t j = blockIdx.x; j < ncols; j += gridDim.x) {
+        const double* col = src + j * lds;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            const double x = col[i];
+            if (NARROW) dst[j * ld + i] = (float)x;
+            mx = nanmax(mx, fabs(x));
+            ss += x * x;
+        }
+    }
+    mx = block_max(mx, red);
+    ss = block_sum(ss, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mx;
+        part[2 * blockIdx.x + 1] = ss;
+    }
+}
+
 // Plain 16-byte copy (counter and bandwidth calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const dbl2* __restrict__ src, dbl2* __restrict__ dst, int64_t nv) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -320,6 +352,13 @@ struct pdot_ctx {
     int32_t resident_blocks = 0;          // step-kernel blocks resident at once
     pdot_solver_state* h_state = nullptr;  // pinned
     double* h_eval = nullptr;              // pinned
+    double* h_pin[2] = {nullptr, nullptr};  // pinned upload staging ring
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    int64_t pin_cols = 0;                   // columns per upload chunk
+    double* ot_part = nullptr;              // per-chunk, per-block (max |C|, sum C^2)
+    int64_t ot_part_bytes = 0;
+    double ot_info[4] = {0, 0, 0, 0};
+    bool ot_ready = false;
     int32_t R = 0, Cb = 0, TC = 0, nstep = 0, NBr = 0, NBc = 0, nred = 0, sqchunk = 0;
     int32_t cb_hint = 0;
     int32_t interleave = 1;
@@ -346,8 +385,12 @@ int fail(pdot_ctx* ctx, int code, const std::string& msg) {
         }                                                                                              \
     } while (0)
 
-int rows_per_block(int32_t storage) { return kWaves * kWave * (storage == PDOT_STORAGE_F64 ? 2 : 4); }
-static_assert(kWaves >= 1 && kWaves <= 16, "step kernel waves");
+int step_waves(int32_t storage) {
+    return storage == PDOT_STORAGE_F64 ? StepShape<double>::kWaves : StepShape<float>::kWaves;
+}
+int rows_per_block(int32_t storage) { return step_waves(storage) * kWave * (storage == PDOT_STORAGE_F64 ? 2 : 4); }
+static_assert(StepShape<double>::kWaves >= 1 && StepShape<double>::kWaves <= 16, "step kernel waves");
+static_assert(StepShape<float>::kWaves >= 1 && StepShape<float>::kWaves <= 16, "step kernel waves");
 
 int dev_alloc(pdot_ctx* ctx, void** p, size_t bytes) {
     if (bytes == 0) bytes = 8;
@@ -456,13 +499,13 @@ ReduceParams reduce_params(pdot_ctx* ctx) {
 
 hipError_t launch_step(pdot_ctx* ctx, bool eval) {
     StepParams p = step_params(ctx);
-    dim3 grid(ctx->nstep), block(kBlock);
+    dim3 grid(ctx->nstep), block(step_waves(ctx->storage) * kWave);
     if (ctx->storage == PDOT_STORAGE_F64) {
-        if (eval) hipLaunchKernelGGL((step_kernel<double, true, kUnroll>), grid, block, 0, ctx->stream, p);
-        else hipLaunchKernelGGL((step_kernel<double, false, kUnroll>), grid, block, 0, ctx->stream, p);
+        if (eval) hipLaunchKernelGGL((step_kernel<double, true>), grid, block, 0, ctx->stream, p);
+        else hipLaunchKernelGGL((step_kernel<double, false>), grid, block, 0, ctx->stream, p);
     } else {
-        if (eval) hipLaunchKernelGGL((step_kernel<float, true, kUnroll>), grid, block, 0, ctx->stream, p);
-        else hipLaunchKernelGGL((step_kernel<float, false, kUnroll>), grid, block, 0, ctx->stream, p);
+        if (eval) hipLaunchKernelGGL((step_kernel<float, true>), grid, block, 0, ctx->stream, p);
+        else hipLaunchKernelGGL((step_kernel<float, false>), grid, block, 0, ctx->stream, p);
     }
     return hipGetLastError();
 }
@@ -528,6 +571,45 @@ int ensure_stage(pdot_ctx* ctx) {
     return dev_alloc(ctx, (void**)&ctx->stage, sizeof(double) * (size_t)(cols * ctx->n));
 }
 
+// Pinned staging ring for uploads: two buffers of up to 64 MiB of whole
+// columns.
+int ensure_pinned(pdot_ctx* ctx) {
+    if (ctx->h_pin[0]) return PDOT_OK;
+    const int64_t max_bytes = (int64_t)64 << 20;
+    int64_t cols = max_bytes / (8 * ctx->n);
+    if (cols < 1) cols = 1;
+    if (cols > ctx->m) cols = ctx->m;
+    ctx->pin_cols = cols;
+    for (int k = 0; k < 2; ++k) {
+        HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_pin[k], sizeof(double) * (size_t)(cols * ctx->n), hipHostMallocDefault));
+        HIPCHK(ctx, hipEventCreateWithFlags(&ctx->pin_ev[k], hipEventDisableTiming));
+    }
+    return PDOT_OK;
+}
+
+// memcpy split over host threads (pageable source into pinned staging).
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    const size_t min_piece = (size_t)4 << 20;
+    unsigned hw = std::thread::hardware_concurrency();
+    size_t nt = hw == 0 ? 1 : (hw < 8 ? hw : 8);
+    if (bytes / min_piece < nt) nt = bytes / min_piece > 0 ? bytes / min_piece : 1;
+    if (nt <= 1) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t piece = (bytes / nt + 63) / 64 * 64;
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nt; ++t) {
+        const size_t lo = t * piece;
+        if (lo >= bytes) break;
+        const size_t len = (lo + piece < bytes) ? piece : bytes - lo;
+        pool.emplace_back([=]() { std::memcpy(static_cast<char*>(dst) + lo, static_cast<const char*>(src) + lo, len); });
+    }
+    for (auto& th : pool) th.join();
+}
+
+double host_nanmax(double acc, double t) { return (acc != acc || t != t || t > acc) ? (acc != acc ? acc : t) : acc; }
+
 }  // namespace
 
 extern "C" {
@@ -569,9 +651,11 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     int ncu = 0, per_cu = 0;
     HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
     if (storage == PDOT_STORAGE_F64)
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kUnroll>, kBlock, 0));
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false>,
+                                                                  StepShape<double>::kWaves * kWave, 0));
     else
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kUnroll>, kBlock, 0));
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false>,
+                                                                  StepShape<float>::kWaves * kWave, 0));
     ctx->resident_blocks = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
     if ((rc = setup_geometry(ctx))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -594,6 +678,11 @@ void pdot_destroy(pdot_ctx* ctx) {
     }
     if (ctx->h_state) hipHostFree(ctx->h_state);
     if (ctx->h_eval) hipHostFree(ctx->h_eval);
+    for (int k = 0; k < 2; ++k) {
+        if (ctx->h_pin[k]) hipHostFree(ctx->h_pin[k]);
+        if (ctx->pin_ev[k]) hipEventDestroy(ctx->pin_ev[k]);
+    }
+    if (ctx->ot_part) hipFree(ctx->ot_part);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -605,24 +694,83 @@ int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b
     if (!C || !a || !b) return fail(ctx, PDOT_E_ARG, "pdot_upload: null pointer");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const int64_t n = ctx->n, m = ctx->m, ld = ctx->ld;
-    if (ctx->storage == PDOT_STORAGE_F64) {
-        HIPCHK(ctx, hipMemcpy2DAsync(ctx->C, ld * 8, C, n * 8, n * 8, m, hipMemcpyHostToDevice, ctx->stream));
-    } else {
-        int rc = ensure_stage(ctx);
-        if (rc) return rc;
-        for (int64_t j0 = 0; j0 < m; j0 += ctx->stage_cols) {
-            const int64_t nc = (j0 + ctx->stage_cols < m) ? ctx->stage_cols : m - j0;
-            HIPCHK(ctx, hipMemcpyAsync(ctx->stage, C + j0 * n, sizeof(double) * n * nc, hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(narrow_kernel<float>, dim3(2048), dim3(256), 0, ctx->stream, ctx->stage,
-                               static_cast<float*>(ctx->C) + j0 * ld, n, ld, nc);
-            HIPCHK(ctx, hipGetLastError());
+    const bool f32 = ctx->storage == PDOT_STORAGE_F32;
+    ctx->ot_ready = false;
+    int rc;
+    if ((rc = ensure_pinned(ctx))) return rc;
+    if (f32 && (rc = ensure_stage(ctx))) return rc;
+    const int64_t cols = f32 ? (ctx->pin_cols < ctx->stage_cols ? ctx->pin_cols : ctx->stage_cols) : ctx->pin_cols;
+    const int64_t nchunks = (m + cols - 1) / cols;
+    const int64_t G = cols < kIngestBlocks ? cols : kIngestBlocks;
+    if (ctx->ot_part) {
+        hipFree(ctx->ot_part);
+        ctx->bytes -= ctx->ot_part_bytes;
+        ctx->ot_part = nullptr;
+    }
+    ctx->ot_part_bytes = (int64_t)(sizeof(double) * 2 * G * nchunks);
+    if ((rc = dev_alloc(ctx, (void**)&ctx->ot_part, (size_t)ctx->ot_part_bytes))) return rc;
+
+    // Pipeline: the host copies chunk k into pinned buffer k % 2 while the DMA
+    // of chunk k - 1 runs; a buffer is refilled once its copy has completed.
+    bool used[2] = {false, false};
+    for (int64_t k = 0; k < nchunks; ++k) {
+        const int64_t j0 = k * cols;
+        const int64_t nc = (j0 + cols < m) ? cols : m - j0;
+        const int b2 = (int)(k & 1);
+        if (used[b2]) HIPCHK(ctx, hipEventSynchronize(ctx->pin_ev[b2]));
+        parallel_copy(ctx->h_pin[b2], C + j0 * n, sizeof(double) * (size_t)(n * nc));
+        double* part = ctx->ot_part + 2 * G * k;
+        if (!f32) {
+            double* dst = static_cast<double*>(ctx->C) + j0 * ld;
+            HIPCHK(ctx, hipMemcpy2DAsync(dst, ld * 8, ctx->h_pin[b2], n * 8, n * 8, nc, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(ctx, hipEventRecord(ctx->pin_ev[b2], ctx->stream));
+            hipLaunchKernelGGL(ingest_kernel<false>, dim3((unsigned)G), dim3(256), 0, ctx->stream, dst, ld,
+                               (float*)nullptr, ld, n, nc, part);
+        } else {
+            HIPCHK(ctx, hipMemcpyAsync(ctx->stage, ctx->h_pin[b2], sizeof(double) * (size_t)(n * nc), hipMemcpyHostToDevice,
+                                       ctx->stream));
+            HIPCHK(ctx, hipEventRecord(ctx->pin_ev[b2], ctx->stream));
+            hipLaunchKernelGGL(ingest_kernel<true>, dim3((unsigned)G), dim3(256), 0, ctx->stream, ctx->stage, n,
+                               static_cast<float*>(ctx->C) + j0 * ld, ld, n, nc, part);
         }
+        HIPCHK(ctx, hipGetLastError());
+        used[b2] = true;
     }
     HIPCHK(ctx, hipMemsetAsync(ctx->a, 0, sizeof(double) * ld, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->a, a, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipMemcpyAsync(ctx->b, b, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<double> parts((size_t)(2 * G * nchunks));
+    HIPCHK(ctx, hipMemcpyAsync(parts.data(), ctx->ot_part, sizeof(double) * parts.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+
+    // cached_ot_info, src/termination.jl:94-101, in a fixed order
+    double cmax = 0.0, csq = 0.0;
+    for (size_t t = 0; t < parts.size(); t += 2) {
+        cmax = host_nanmax(cmax, parts[t]);
+        csq += parts[t + 1];
+    }
+    double qmax = 0.0, qsq = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        qmax = host_nanmax(qmax, std::fabs(a[i]));
+        qsq += a[i] * a[i];
+    }
+    for (int64_t j = 0; j < m; ++j) {
+        qmax = host_nanmax(qmax, std::fabs(b[j]));
+        qsq += b[j] * b[j];
+    }
+    ctx->ot_info[0] = cmax;
+    ctx->ot_info[1] = qmax;
+    ctx->ot_info[2] = std::sqrt(csq);
+    ctx->ot_info[3] = std::sqrt(qsq);
+    ctx->ot_ready = true;
     ctx->uploaded = true;
+    return PDOT_OK;
+}
+
+int pdot_cached_ot_info(pdot_ctx* ctx, double* out4) {
+    if (!ctx || !out4) return PDOT_E_ARG;
+    if (!ctx->ot_ready) return fail(ctx, PDOT_E_STATE, "pdot_cached_ot_info before pdot_upload");
+    for (int k = 0; k < 4; ++k) out4[k] = ctx->ot_info[k];
     return PDOT_OK;
 }
 
@@ -816,6 +964,34 @@ int pdot_download(pdot_ctx* ctx, int32_t which, double* X, double* u, double* v)
     if (v) HIPCHK(ctx, hipMemcpyAsync(v, ctx->s.v[slot], sizeof(double) * m, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return PDOT_OK;
+}
+
+int pdot_round(pdot_ctx* ctx, int32_t which, double* X_rounded) {
+    if (!ctx || !X_rounded) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_round before pdot_set_start");
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    int slot;
+    if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
+    else if (which == PDOT_POINT_AVERAGE) {
+        if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_round: the average is empty");
+        slot = st.avg_slot;
+    } else return fail(ctx, PDOT_E_ARG, "pdot_round: unknown point");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    double* Y = nullptr;
+    if ((rc = dev_alloc(ctx, (void**)&Y, sizeof(double) * (size_t)(ctx->n * ctx->m)))) return rc;
+    ctx->bytes -= (int64_t)(sizeof(double) * (size_t)(ctx->n * ctx->m));  // released below
+    rc = pdot_round_plan(ctx->stream, ctx->s.X[slot], ctx->storage, ctx->ld, ctx->n, ctx->m, ctx->a, ctx->b, Y,
+                         ctx->n, &ctx->err);
+    if (rc == PDOT_OK) {
+        hipError_t e = hipMemcpyAsync(X_rounded, Y, sizeof(double) * (size_t)(ctx->n * ctx->m), hipMemcpyDeviceToHost,
+                                      ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) rc = fail(ctx, PDOT_E_HIP, std::string("pdot_round: copy: ") + hipGetErrorString(e));
+    }
+    hipFree(Y);
+    return rc;
 }
 
 int pdot_time_steps(pdot_ctx* ctx, int64_t count, double* step_kernel_ms, double* total_ms) {

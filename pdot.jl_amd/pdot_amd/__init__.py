@@ -34,3 +34,6 @@ def default_parameters(tolerance: float = 1e-4, *, iteration_limit: int = 2**31 
         time_sec_limit=time_sec_limit, iteration_limit=iteration_limit, kkt_matrix_pass_limit=float("inf"))
     return PdhgParameters(1.0, False, verbosity, record_iteration_stats, 128, termination_params, restart_params,
                           AdaptiveStepsizeParams(2, 1))
+
+
+from .output import rounding, solve_instance_and_output, write_vector_to_file  # noqa: E402

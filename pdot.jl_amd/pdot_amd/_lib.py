@@ -95,6 +95,7 @@ SIGNATURES = {
     "pdot_destroy": (None, [_ctx_p]),
     "pdot_last_error": (ctypes.c_char_p, [_ctx_p]),
     "pdot_upload": (ctypes.c_int, [_ctx_p, _c_double_p, _c_double_p, _c_double_p]),
+    "pdot_cached_ot_info": (ctypes.c_int, [_ctx_p, _c_double_p]),
     "pdot_set_start": (ctypes.c_int, [_ctx_p, _c_double_p, _c_double_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "pdot_run_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, ctypes.POINTER(SolverState)]),
     "pdot_eval": (ctypes.c_int, [_ctx_p, ctypes.POINTER(PointStats), ctypes.POINTER(PointStats), ctypes.POINTER(SolverState)]),
@@ -109,6 +110,11 @@ SIGNATURES = {
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
     "pdot_copy_probe": (ctypes.c_int, [_ctx_p, ctypes.c_int64, ctypes.c_int32, _c_double_p]),
+    # rounding and output
+    "pdot_round": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p]),
+    "pdot_round_matrix": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _c_double_p, _c_double_p, _c_double_p, ctypes.c_char_p, ctypes.c_int64]),
+    "pdot_write_julia_vector": (ctypes.c_int, [ctypes.c_char_p, _c_double_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
+    "pdot_format_julia_float": (ctypes.c_int, [ctypes.c_double, ctypes.c_char_p, ctypes.c_int64]),
     # batches
     "pdot_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int32, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.c_int32, ctypes.c_int32]),
     "pdot_batch_destroy": (None, [_ctx_p]),
@@ -259,6 +265,21 @@ class DeviceSolver:
         xp = X.ctypes.data_as(_c_double_p) if X is not None else None
         self._check(self.lib.pdot_download(self.h, int(which), xp, _ptr(u), _ptr(v)), "pdot_download")
         return X, u, v
+
+    def cached_ot_info(self):
+        """termination.cached_ot_info of the uploaded problem, from the device
+        upload pass (src/termination.jl:94-101)."""
+        from .termination import CachedOTInfo
+        out = np.zeros(4)
+        self._check(self.lib.pdot_cached_ot_info(self.h, _ptr(out)), "pdot_cached_ot_info")
+        return CachedOTInfo(*(float(x) for x in out))
+
+    def round(self, which: int = POINT_AVERAGE) -> np.ndarray:
+        """Feasibility rounding of the resident current or average plan
+        (src/utils_gpu.jl:296-318) computed on the device; FP64 n x m."""
+        X = np.empty((self.n, self.m), dtype=np.float64, order="F")
+        self._check(self.lib.pdot_round(self.h, int(which), X.ctypes.data_as(_c_double_p)), "pdot_round")
+        return X
 
     def time_steps(self, count: int):
         ks, tt = ctypes.c_double(), ctypes.c_double()

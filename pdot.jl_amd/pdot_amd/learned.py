@@ -197,16 +197,44 @@ def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
         return z["params"].astype(np.float32)
 
 
+def zero_start_relative_kkt(problem) -> float:
+    """Relative KKT error of the zero start at its first evaluation, in closed
+    form: X = 0, u = v = 0 (so G = 0) leave only the primal residual
+    ||[a; b]||_2, relative to 1 + ||[a; b]||_2 (src/utils_gpu.jl:91-124)."""
+    q = float(np.sqrt(np.dot(problem.source_distribution, problem.source_distribution)
+                      + np.dot(problem.target_distribution, problem.target_distribution)))
+    return q / (1.0 + q)
+
+
 def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True):
-    """optimize() from the network's duals; if that solve does not reach
-    OPTIMAL, the zero-start solve is run and returned (the fallback).
+    """optimize() from the network's duals, with zero-start PDHG as the
+    fallback and the solver's own KKT check as the judge:
+
+    1. the predicted start is evaluated (the first evaluation of a solve
+       limited to one iteration); if its relative KKT error is not below the
+       zero start's, the solve runs from zero;
+    2. a warm-started solve that does not reach OPTIMAL is rerun from zero.
+
     Returns (output, used_warm_start, predict_seconds)."""
+    from dataclasses import replace
     from . import solver as S
     from .solve_log import TerminationReason
     t0 = time.time()
     u0, v0 = net.predict_duals(x, y, problem.source_distribution, problem.target_distribution)
     pred = time.time() - t0
-    out = S.optimize(params, problem, storage=storage, warm_start=(u0, v0))
-    if out.termination_reason == TerminationReason.TERMINATION_REASON_OPTIMAL or not fallback:
-        return out, True, pred
-    return S.optimize(params, problem, storage=storage), False, pred
+    n, m = problem.cost_matrix.shape
+    dev = _lib.DeviceSolver(n, m, storage=storage)
+    try:
+        dev.upload(problem.cost_matrix, problem.source_distribution, problem.target_distribution)
+        probe = replace(params, termination_criteria=replace(params.termination_criteria, iteration_limit=1),
+                        verbosity=0)
+        first = S.optimize(probe, problem, storage=storage, warm_start=(u0, v0), solver=dev)
+        warm_kkt = S.relative_kkt_error(first.iteration_stats[0].convergence_information[0])
+        if not (warm_kkt < zero_start_relative_kkt(problem)):
+            return S.optimize(params, problem, storage=storage, solver=dev), False, pred
+        out = S.optimize(params, problem, storage=storage, warm_start=(u0, v0), solver=dev)
+        if out.termination_reason == TerminationReason.TERMINATION_REASON_OPTIMAL or not fallback:
+            return out, True, pred
+        return S.optimize(params, problem, storage=storage, solver=dev), False, pred
+    finally:
+        dev.close()

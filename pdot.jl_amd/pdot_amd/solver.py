@@ -265,7 +265,6 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
     scalars in FP64).  warm_start: optional (u0, v0) initial duals; X starts at
     0 and G at (C - u0) - v0.  solver: reuse a DeviceSolver whose problem is
     already uploaded (the upload is then skipped)."""
-    ot_cache = cached_ot_info(original_problem)
     nrow, ncol = original_problem.cost_matrix.shape
     if params.primal_importance <= 0 or not math.isfinite(params.primal_importance):
         raise ValueError("primal_importance must be positive and finite")
@@ -281,6 +280,8 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
         raise ValueError("solver was created for another problem size")
     solver.use_graphs(use_graphs)
     upload_sec = time.time() - t_up
+    # termination.jl:94-101, from the device upload pass (no host pass over C)
+    ot_cache = solver.cached_ot_info()
 
     policy, re, ge, step_size, primal_weight = initial_step_and_weight(params, nrow, ncol, ot_cache)
     u0 = v0 = None
@@ -308,6 +309,7 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
         if reason is not None:
             solver.discard()
             which = _lib.POINT_CURRENT if st.avg_count == 0 else _lib.POINT_AVERAGE
+            solver.output_point = which  # the returned point stays resident (rounding on the device)
             X, u, v = solver.download(which)
             if profile is not None:
                 profile.upload_sec = upload_sec

@@ -9,9 +9,9 @@
    (u + t, v - t) invariance, are the targets.
 2. Training: Adam on the GPU through pdot_mlp, one instance per step.
 3. Held-out evaluation on instances of the same family at n = m = 4096 with
-   seeds disjoint from training: iterations and time to 1e-4 and 1e-6 from
-   the zero start and from the predicted duals (same solver, same
-   parameters; only the starting iterate differs).
+   seeds disjoint from training: iterations and time to 1e-4 from the zero
+   start and from the predicted duals (same solver, same parameters; only
+   the starting iterate differs).
 """
 from __future__ import annotations
 
@@ -51,30 +51,39 @@ def generate_training_data(count, n_min, n_max, tol, seed, batch_size=512, itera
     return samples
 
 
-def evaluate(net, count, n, tols=(1e-4, 1e-6), storage=_lib.STORAGE_F64, log=print):
+def evaluate(net, count, n, tols=(1e-4,), storage=_lib.STORAGE_F64, log=print):
+    """Held-out comparison: iterations and time to each tolerance from the
+    zero start and through learned.solve_warm (prediction, KKT judge,
+    fallback), same solver and parameters.  Warm solves are capped at 20x the
+    zero start's iterations (then reported as not reaching the tolerance)."""
+    from dataclasses import replace
     rows = []
     for k in range(count):
         inst = instances.gmm_dirichlet(n, seed=EVAL_SEED0 + k)
         p = inst.problem
         row = {"seed": EVAL_SEED0 + k}
+        dev = _lib.DeviceSolver(n, n, storage=storage)
+        dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
         t0 = time.time()
         u0, v0 = net.predict_duals(inst.x, inst.y, p.source_distribution, p.target_distribution)
         row["predict_sec"] = time.time() - t0
-        dev = _lib.DeviceSolver(n, n, storage=storage)
-        dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
         for tol in tols:
-            for name, ws in (("zero", None), ("warm", (u0, v0))):
-                prof = SolveProfile()
-                out = optimize(default_parameters(tol, record_iteration_stats=False), p, storage=storage,
-                               warm_start=ws, solver=dev, profile=prof)
-                ci = out.iteration_stats[-1].convergence_information[0]
-                row[f"{name}_{tol:g}"] = {"iterations": out.iteration_count, "seconds": prof.solve_sec,
-                                          "termination": out.termination_string,
-                                          "objective": ci.primal_objective, "rel_kkt": relative_kkt_error(ci)}
+            params = default_parameters(tol, record_iteration_stats=False)
+            prof = SolveProfile()
+            out = optimize(params, p, storage=storage, solver=dev, profile=prof)
+            ci = out.iteration_stats[-1].convergence_information[0] if out.iteration_stats else None
+            row[f"zero_{tol:g}"] = {"iterations": out.iteration_count, "seconds": prof.solve_sec,
+                                     "termination": out.termination_string}
+            log(f"eval {k} tol {tol:g} zero: {out.iteration_count} it {prof.solve_sec:.2f}s {out.termination_string}")
+            cap = replace(params, termination_criteria=replace(params.termination_criteria,
+                                                               iteration_limit=max(1000, 20 * out.iteration_count)))
+            prof = SolveProfile()
+            out = optimize(cap, p, storage=storage, warm_start=(u0, v0), solver=dev, profile=prof)
+            row[f"warm_{tol:g}"] = {"iterations": out.iteration_count, "seconds": prof.solve_sec,
+                                     "termination": out.termination_string}
+            log(f"eval {k} tol {tol:g} warm: {out.iteration_count} it {prof.solve_sec:.2f}s {out.termination_string}")
         dev.close()
         rows.append(row)
-        log(f"eval {k}: " + ", ".join(f"{key} {row[key]['iterations']}it {row[key]['seconds']:.2f}s"
-                                      for key in row if key.startswith(("zero", "warm"))))
     summary = {}
     for tol in tols:
         z = np.array([r[f"zero_{tol:g}"]["iterations"] for r in rows], dtype=float)
@@ -82,7 +91,7 @@ def evaluate(net, count, n, tols=(1e-4, 1e-6), storage=_lib.STORAGE_F64, log=pri
         zt = np.array([r[f"zero_{tol:g}"]["seconds"] for r in rows])
         wt = np.array([r[f"warm_{tol:g}"]["seconds"] + r["predict_sec"] for r in rows])
         summary[f"{tol:g}"] = {"iterations_zero_median": float(np.median(z)), "iterations_warm_median": float(np.median(w)),
-                               "iteration_ratio_median": float(np.median(z / w)),
+                               "iteration_ratio_median": float(np.median(z / np.maximum(w, 1))),
                                "time_zero_total": float(zt.sum()), "time_warm_total_incl_predict": float(wt.sum()),
                                "time_ratio_total": float(zt.sum() / wt.sum()),
                                "warm_optimal": int(sum(r[f"warm_{tol:g}"]["termination"] == "OPTIMAL" for r in rows))}
@@ -122,7 +131,7 @@ def main(argv=None) -> int:
     log(f"validation mse {val_loss:.4e} (predicting zero: {zero:.4e})")
     params = net.get_params()
     learned.save_params(args.out, params, {"train_instances": len(trn), "val_mse": val_loss, "epochs": args.epochs})
-    rows, summary = evaluate(net, args.eval, args.eval_n, log=log)
+    rows, summary = evaluate(net, args.eval, args.eval_n, log=log) if args.eval > 0 else ([], {})
     report = {"data_sec": data_sec, "train_sec": train_sec, "train_instances": len(trn), "val_instances": len(val),
               "train_loss_history": hist, "val_mse": val_loss, "val_mse_zero_prediction": zero,
               "eval_n": args.eval_n, "eval": rows, "summary": summary}

@@ -245,3 +245,47 @@ def test_rounding_is_feasible_and_close():
     np.testing.assert_allclose(X.sum(axis=1), prob.source_distribution, atol=1e-15)
     np.testing.assert_allclose(X.sum(axis=0), prob.target_distribution, atol=1e-15)
     assert X.min() >= -1e-15  # the rank-1 correction can leave rounding-level negatives
+
+
+def test_rounding_known_answer_by_hand():
+    """2 x 2 case worked by hand through src/utils_gpu.jl:296-318:
+    X = [.4 .2; 0 .1], a = b = (.5, .5): s = (5/6, 1), t = (1, 1),
+    e_r = (0, .4)/.4, e_c = (1/6, 7/30), Y = [1/3 1/6; 1/6 1/3]."""
+    X = np.array([[0.4, 0.2], [0.0, 0.1]], order="F")
+    Y = O.rounding(np.array([0.5, 0.5]), np.array([0.5, 0.5]), X)
+    np.testing.assert_allclose(Y, [[1 / 3, 1 / 6], [1 / 6, 1 / 3]], rtol=0, atol=2e-16)
+
+
+def test_rounding_leaves_a_feasible_plan_unchanged():
+    """A feasible plan with dyadic entries (exact sums): s = t = 1 and the
+    residuals are exactly 0, so the output is bit-identical."""
+    rng = np.random.default_rng(4)
+    a = rng.integers(1, 64, 37).astype(float)
+    b = rng.integers(1, 64, 23).astype(float)
+    a /= 2.0 ** np.ceil(np.log2(a.sum()))
+    a[0] += 1.0 - a.sum()
+    b /= 2.0 ** np.ceil(np.log2(b.sum()))
+    b[0] += 1.0 - b.sum()
+    X = np.asfortranarray(np.outer(a, b))
+    assert np.array_equal(X.sum(axis=1), a) and np.array_equal(X.sum(axis=0), b)
+    assert np.array_equal(O.rounding(a, b, X), X)
+
+
+def test_rounding_error_bound_and_zero_rows():
+    """Altschuler, Weed, Rigollet (2017), Lemma 7: the rounded plan Y is
+    feasible and ||Y - X||_1 <= 2 (||r(X) - a||_1 + ||c(X) - b||_1).  A row
+    and a column of zeros (0/0 -> 0 scaling) and masses above the plan's sums
+    (ratio capped at 1) are included."""
+    rng = np.random.default_rng(5)
+    for n, m in ((1, 9), (9, 1), (17, 31), (64, 50)):
+        a = rng.random(n); a /= a.sum()
+        b = rng.random(m); b /= b.sum()
+        X = np.asfortranarray(rng.random((n, m)) * rng.choice([0.5, 2.0], size=(n, 1)) / (n * m))
+        X[n // 2, :] = 0.0
+        X[:, m // 2] = 0.0
+        Y = O.rounding(a, b, X)
+        np.testing.assert_allclose(Y.sum(axis=1), a, atol=1e-15)
+        np.testing.assert_allclose(Y.sum(axis=0), b, atol=1e-15)
+        bound = 2 * (np.abs(X.sum(axis=1) - a).sum() + np.abs(X.sum(axis=0) - b).sum())
+        assert np.abs(Y - X).sum() <= bound * (1 + 1e-12)
+        assert Y.min() >= -1e-15
