@@ -84,15 +84,25 @@ def cpu_baseline(prob, iters: int):
                       f"{len(r['stats'])} evaluations)"}
 
 
-def solve_to_tol(prob, storage, tol, dev=None):
+def solve_to_tol(prob, storage, tol, dev=None, do_round=True):
     import pdot_amd as P
     prof = P.SolveProfile()
     out = P.optimize(P.default_parameters(tol, verbosity=0, record_iteration_stats=False), prob,
                      storage=storage, solver=dev, profile=prof)
     ci = out.iteration_stats[-1].convergence_information[0]
-    return {"seconds": prof.solve_sec, "iterations": out.iteration_count,
-            "termination": out.termination_string, "objective": ci.primal_objective,
-            "rel_kkt": P.relative_kkt_error(ci), "evaluations": prof.evaluations, "restarts": prof.restarts}
+    res = {"seconds": prof.solve_sec, "iterations": out.iteration_count,
+           "termination": out.termination_string, "objective": ci.primal_objective,
+           "rel_kkt": P.relative_kkt_error(ci), "evaluations": prof.evaluations, "restarts": prof.restarts}
+    if dev is not None and do_round:
+        # output post-processing: feasibility rounding of the resident solution on the device
+        # (four reads of X, one FP64 write) plus the device-to-host copy of the rounded plan
+        t0 = time.time()
+        Xr = dev.round(dev.output_point)
+        res["rounding_sec_incl_copy"] = time.time() - t0
+        if hasattr(prob, "cost_matrix"):
+            res["rounded_objective"] = float((prob.cost_matrix * Xr).sum())
+        del Xr
+    return res
 
 
 def bench_l1_mixture(args) -> int:
@@ -118,9 +128,23 @@ def bench_l1_mixture(args) -> int:
         res = {"upload_sec": up, "ms_per_step": tt / args.steps, "step_kernel_GBs": alg / k_avg / 1e9,
                "device_GB": dev.device_bytes() / 1e9}
         if not args.no_solve:
-            res["solve"] = solve_to_tol(prob, storage, 1e-4, dev=dev)
+            res["solve"] = solve_to_tol(prob, storage, 1e-4, dev=dev, do_round=False)
         out[name] = res
         dev.close()
+    # point-cloud cost form, FP32 storage: C never stored, 4 n*m streams per step
+    pts = instances.l1_mixture_3d_points(n)
+    dev = _lib.DeviceSolver(n, m, storage=_lib.STORAGE_F32)
+    t0 = time.time()
+    dev.upload_points(pts)
+    up = time.time() - t0
+    ks, tt, st = measure_steps(dev, args.steps, args.warmup)
+    alg = 4 * n * m * 4 + 80 * (n + m)
+    res = {"upload_sec": up, "ms_per_step": tt / args.steps, "step_kernel_GBs": alg / (ks / args.steps * 1e-3) / 1e9,
+           "device_GB": dev.device_bytes() / 1e9}
+    if not args.no_solve:
+        res["solve"] = solve_to_tol(pts, _lib.STORAGE_F32, 1e-4, dev=dev, do_round=False)
+    out["f32_point_cloud"] = res
+    dev.close()
     if not args.no_solve:
         out["value"] = out["f32"]["solve"]["seconds"]
         out["objective_rel_err_f32_vs_f64"] = (abs(out["f32"]["solve"]["objective"] - out["f64"]["solve"]["objective"])
@@ -248,7 +272,9 @@ def main() -> int:
     devs = {}
     for storage, name, elem in ((_lib.STORAGE_F64, "f64", 8), (_lib.STORAGE_F32, "f32", 4)):
         dev = _lib.DeviceSolver(n, m, storage=storage)
+        t0 = time.time()
         dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+        up = time.time() - t0
         ks, tt, st = measure_steps(dev, args.steps, args.warmup)
         alg = algorithmic_bytes_per_step(n, m, elem)
         k_avg = ks / args.steps * 1e-3
@@ -257,8 +283,25 @@ def main() -> int:
             "step_kernel_us": k_avg * 1e6, "step_kernel_GBs": alg / k_avg / 1e9,
             "iteration_GBs": alg / (tt / args.steps * 1e-3) / 1e9, "bytes_per_step": alg,
             "launch": dev.get_launch(), "numerical_error": int(st.numerical_error),
+            "upload_sec": up, "upload_GBs_pcie": 8 * n * m / up / 1e9,
         }
         devs[name] = dev
+
+    # point-cloud cost form (C formed in registers, never stored): 4 n*m streams per step
+    points = {}
+    pts = instances.gaussian_shift_points(n, m)
+    for storage, name, elem in ((_lib.STORAGE_F64, "f64", 8), (_lib.STORAGE_F32, "f32", 4)):
+        dev = _lib.DeviceSolver(n, m, storage=storage)
+        dev.upload_points(pts)
+        ks, tt, st = measure_steps(dev, args.steps, args.warmup)
+        alg = 4 * n * m * elem + 80 * (n + m)
+        k_avg = ks / args.steps * 1e-3
+        points[name] = {"ms_per_step": tt / args.steps, "iters_per_s": args.steps / (tt * 1e-3),
+                        "step_kernel_us": k_avg * 1e6, "step_kernel_GBs": alg / k_avg / 1e9, "bytes_per_step": alg,
+                        "device_GB": dev.device_bytes() / 1e9}
+        if not args.no_solve:
+            points[name]["time_to_1e-4"] = solve_to_tol(pts, storage, 1e-4, dev=dev)
+        dev.close()
 
     solves = {}
     if not args.no_solve:
@@ -304,6 +347,7 @@ def main() -> int:
         "iteration_GBs": f64["iteration_GBs"],
         "f32_storage": results["f32"],
         "time_to_1e-4": solves or None,
+        "point_cloud_cost": points,
     }
     if copy_probe:
         line["copy_probe"] = copy_probe

@@ -37,6 +37,7 @@ enum {
 enum { PDOT_STORAGE_F64 = 0, PDOT_STORAGE_F32 = 1 };
 enum { PDOT_POLICY_ADAPTIVE = 0, PDOT_POLICY_CONSTANT = 1 };
 enum { PDOT_POINT_CURRENT = 0, PDOT_POINT_AVERAGE = 1 };
+enum { PDOT_COST_SQEUCLID = 1, PDOT_COST_L1 = 2 };
 
 /* Device-resident solver scalars: the scalar fields of CuPdhgSolverState
  * (src/solver_gpu.jl:23-36) and of CuSolutionWeightedAverage
@@ -102,6 +103,17 @@ const char* pdot_last_error(const pdot_ctx* ctx);
  * previous one); FP32 storage is narrowed on the device; the same pass
  * computes the norms of cached_ot_info. */
 int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b);
+
+/* Point-cloud form of the problem, an extension beyond the reference's dense
+ * Matrix{Float64} contract: the cost is never stored but formed in registers
+ * each pass, C_ij = (sum_d f(y_jd - x_id)) / s with f(t) = t*t
+ * (PDOT_COST_SQEUCLID) or |t| (PDOT_COST_L1), summed from 0 in d order, and
+ * s = max_ij of the raw sum if `normalize`, else 1 -- bit for bit the matrix a
+ * dense caller builds that way, so results equal those of pdot_upload of
+ * that matrix.  x: n x dim, y: m x dim (row-major), dim 1..3.  One n*m
+ * stream per pass fewer than the dense form; no n*m allocation for C. */
+int pdot_upload_points(pdot_ctx* ctx, int32_t cost_kind, int32_t dim, const double* x, const double* y,
+                       int32_t normalize, const double* a, const double* b);
 
 /* cached_ot_info (src/termination.jl:94-101) of the uploaded problem, in the
  * CachedOTInfo field order: ||C||_inf, ||[a;b]||_inf, ||C||_F, ||[a;b]||_2.

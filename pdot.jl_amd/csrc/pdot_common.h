@@ -315,12 +315,36 @@ __device__ __forceinline__ void store_vec<float>(float* p, const double* o) {
 
 // ------------------------------------------------------------ step tile
 
+// Where the cost C_ij comes from: the stored matrix, or formed in registers
+// from point clouds (source x: dim x n, target y: m x dim) exactly as the
+// dense generators form it, C_ij = (sum_d f(y_jd - x_id)) / cdiv, with
+// f(t) = t*t (squared Euclidean) or |t| (L1), summed from 0 in d order.
+enum { kCostDense = 0, kCostSqEuclid = 1, kCostL1 = 2 };
+constexpr int kMaxDim = 3;
+
+template <int COST>
+__device__ __forceinline__ double point_cost(const double* xi, const double* yj, int dim, double cdiv) {
+    double c = 0.0;
+#pragma unroll
+    for (int d = 0; d < kMaxDim; ++d) {
+        if (d < dim) {
+            const double t = yj[d] - xi[d];
+            c += (COST == kCostSqEuclid) ? t * t : fabs(t);
+        }
+    }
+    return c / cdiv;
+}
+
 // Everything one block needs to advance one tile of one problem by a PDHG
 // pass: rows [row0, row0 + WAVES*64*VEC) of the columns j0 + t*jstep,
 // t < ncols.
 template <typename T>
 struct TileArgs {
-    const T* C;
+    const T* C;          // dense cost (kCostDense)
+    const double* px;    // point costs: source coordinates, dim x n (leading dimension n)
+    const double* py;    // target coordinates, m x dim
+    double cdiv;
+    int dim;
     const T* Xc;   // current point
     const T* Xa;   // running average (read when !avg_empty or EVAL)
     T* Xn;         // new point
@@ -344,7 +368,7 @@ struct TileArgs {
 // evaluation reductions of the input point and of its average.  Called by
 // all threads of the block; thread 0 receives ||Delta||^2 of the tile in
 // *dd_out and the kMatStats evaluation reductions in ev_out.
-template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE>
+template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE, int COST = kCostDense>
 __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)[kChunk], double* red, double* dd_out,
                                           double* ev_out) {
     constexpr int VEC = VecT<T>::N;
@@ -378,6 +402,13 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
         ui[e] = (lane_active && valid[e]) ? a.uc[i0 + e] : 0.0;
         uai[e] = (EVAL && lane_active && valid[e]) ? a.ua[i0 + e] : 0.0;
     }
+    // point costs: this lane's source coordinates stay in registers
+    double xi[VEC][kMaxDim];
+#pragma unroll
+    for (int e = 0; e < VEC; ++e)
+#pragma unroll
+        for (int d = 0; d < kMaxDim; ++d)
+            xi[e][d] = (COST != kCostDense && d < a.dim && lane_active && valid[e]) ? a.px[(int64_t)d * n + i0 + e] : 0.0;
     double rs[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) rs[e] = 0.0;
@@ -394,6 +425,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     struct Batch {
         V xv[UNROLL], cv[UNROLL], av[UNROLL];
         double vj[UNROLL], vaj[UNROLL];
+        double yj[UNROLL][COST == kCostDense ? 1 : kMaxDim];
     };
     auto load_batch = [&](int64_t tb, Batch& B) {
 #pragma unroll
@@ -402,10 +434,14 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             const int64_t j = j0 + (tb + q) * jstep;
             B.vj[q] = colok ? vc[j] : 0.0;
             B.vaj[q] = (EVAL && colok) ? va[j] : 0.0;
+            if constexpr (COST != kCostDense) {
+#pragma unroll
+                for (int d = 0; d < kMaxDim; ++d) B.yj[q][d] = (colok && d < a.dim) ? a.py[j * a.dim + d] : 0.0;
+            }
             if (colok && lane_active) {
                 const int64_t idx = j * ld + i0;
                 B.xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
-                B.cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
+                if constexpr (COST == kCostDense) B.cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
                 if (!avg_empty) B.av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
             }
         }
@@ -421,7 +457,12 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                 const int64_t idx = j * ld + i0;
                 double x[VEC], c[VEC], xa[VEC], xn[VEC], xan[VEC];
                 unpack<V, T>(B.xv[q], x);
-                unpack<V, T>(B.cv[q], c);
+                if constexpr (COST == kCostDense) {
+                    unpack<V, T>(B.cv[q], c);
+                } else {
+#pragma unroll
+                    for (int e = 0; e < VEC; ++e) c[e] = point_cost<COST>(xi[e], B.yj[q], a.dim, a.cdiv);
+                }
                 if (!avg_empty) unpack<V, T>(B.av[q], xa);
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) {

@@ -57,6 +57,10 @@ struct SlotPtrs {
 
 struct StepParams {
     const void* C;
+    const double* px;  // point costs (cost_kind != kCostDense)
+    const double* py;
+    double cdiv;
+    int32_t dim;
     SlotPtrs s;
     const pdot_solver_state* st;
     double* rowpart;
@@ -88,7 +92,7 @@ struct ReduceParams {
 // One PDHG pass over the whole problem: block (rb, cb) owns rows
 // [rb*TR, (rb+1)*TR) of the columns cb, cb+Cb, ... (interleaved) or of the
 // contiguous range [cb*TC, (cb+1)*TC); step_tile does the work.
-template <typename T, bool EVAL>
+template <typename T, bool EVAL, int COST>
 __global__ __launch_bounds__(StepShape<T>::kWaves * kWave) void step_kernel(StepParams p) {
     constexpr int WAVES = StepShape<T>::kWaves;
     constexpr int TR = WAVES * kWave * VecT<T>::N;
@@ -116,6 +120,10 @@ __global__ __launch_bounds__(StepShape<T>::kWaves * kWave) void step_kernel(Step
     const int cs = sst.cur_slot;
     const int as = a.avg_empty ? cs : sst.avg_slot;
     a.C = static_cast<const T*>(p.C);
+    a.px = p.px;
+    a.py = p.py;
+    a.cdiv = p.cdiv;
+    a.dim = p.dim;
     a.Xc = static_cast<const T*>(p.s.X[cs]);
     a.Xa = static_cast<const T*>(p.s.X[as]);
     a.Xn = static_cast<T*>(p.s.X[sst.spare0]);
@@ -138,7 +146,7 @@ __global__ __launch_bounds__(StepShape<T>::kWaves * kWave) void step_kernel(Step
     a.gzero = sst.gzero != 0;
     a.do_step = do_step;
     double dd = 0.0, ev[kMatStats];
-    step_tile<T, EVAL, StepShape<T>::kUnroll, WAVES, StepShape<T>::kPipe>(a, colbuf, red, &dd, ev);
+    step_tile<T, EVAL, StepShape<T>::kUnroll, WAVES, StepShape<T>::kPipe, COST>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
         if (do_step) p.sqpart[blockIdx.x] = dd;
         if (EVAL) {
@@ -300,6 +308,34 @@ __global__ __launch_bounds__(256) void ingest_kernel(const double* __restrict__ 
     }
 }
 
+// cached_ot_info inputs for point costs: the same partials, in the same
+// order, as ingest_kernel over the matrix the points define (one launch per
+// upload chunk of ncols columns from column j0), formed in registers.
+template <int COST>
+__global__ __launch_bounds__(256) void point_stats_kernel(const double* __restrict__ px, const double* __restrict__ py,
+                                                          int64_t n, int dim, double cdiv, int64_t j0, int64_t ncols,
+                                                          double* __restrict__ part) {
+    __shared__ double red[256];
+    double mx = 0.0, ss = 0.0;
+    for (int64_t j = blockIdx.x; j < ncols; j += gridDim.x) {
+        double yj[kMaxDim];
+        for (int d = 0; d < kMaxDim; ++d) yj[d] = d < dim ? py[(j0 + j) * dim + d] : 0.0;
+        for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+            double xi[kMaxDim];
+            for (int d = 0; d < kMaxDim; ++d) xi[d] = d < dim ? px[(int64_t)d * n + i] : 0.0;
+            const double x = point_cost<COST>(xi, yj, dim, cdiv);
+            mx = nanmax(mx, fabs(x));
+            ss += x * x;
+        }
+    }
+    mx = block_max(mx, red);
+    ss = block_sum(ss, red);
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = mx;
+        part[2 * blockIdx.x + 1] = ss;
+    }
+}
+
 // Plain 16-byte copy (counter and bandwidth calibration).
 __global__ __launch_bounds__(256) void copy_kernel(const dbl2* __restrict__ src, dbl2* __restrict__ dst, int64_t nv) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -359,6 +395,11 @@ struct pdot_ctx {
     int64_t ot_part_bytes = 0;
     double ot_info[4] = {0, 0, 0, 0};
     bool ot_ready = false;
+    int32_t cost_kind = kCostDense;         // kCostDense: C stored; else formed from px, py
+    int32_t dim = 0;
+    double cdiv = 1.0;
+    double* px = nullptr;                   // dim x n source coordinates
+    double* py = nullptr;                   // m x dim target coordinates
     int32_t R = 0, Cb = 0, TC = 0, nstep = 0, NBr = 0, NBc = 0, nred = 0, sqchunk = 0;
     int32_t cb_hint = 0;
     int32_t interleave = 1;
@@ -456,6 +497,10 @@ int setup_geometry(pdot_ctx* ctx) {
 StepParams step_params(pdot_ctx* ctx) {
     StepParams p;
     p.C = ctx->C;
+    p.px = ctx->px;
+    p.py = ctx->py;
+    p.cdiv = ctx->cdiv;
+    p.dim = ctx->dim;
     p.s = ctx->s;
     p.st = ctx->stA;
     p.rowpart = ctx->rowpart;
@@ -497,16 +542,25 @@ ReduceParams reduce_params(pdot_ctx* ctx) {
     return p;
 }
 
+template <typename T>
+void launch_step_t(int cost, bool eval, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
+    if (cost == kCostSqEuclid) {
+        if (eval) hipLaunchKernelGGL((step_kernel<T, true, kCostSqEuclid>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((step_kernel<T, false, kCostSqEuclid>), grid, block, 0, stream, p);
+    } else if (cost == kCostL1) {
+        if (eval) hipLaunchKernelGGL((step_kernel<T, true, kCostL1>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((step_kernel<T, false, kCostL1>), grid, block, 0, stream, p);
+    } else {
+        if (eval) hipLaunchKernelGGL((step_kernel<T, true, kCostDense>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((step_kernel<T, false, kCostDense>), grid, block, 0, stream, p);
+    }
+}
+
 hipError_t launch_step(pdot_ctx* ctx, bool eval) {
     StepParams p = step_params(ctx);
     dim3 grid(ctx->nstep), block(step_waves(ctx->storage) * kWave);
-    if (ctx->storage == PDOT_STORAGE_F64) {
-        if (eval) hipLaunchKernelGGL((step_kernel<double, true>), grid, block, 0, ctx->stream, p);
-        else hipLaunchKernelGGL((step_kernel<double, false>), grid, block, 0, ctx->stream, p);
-    } else {
-        if (eval) hipLaunchKernelGGL((step_kernel<float, true>), grid, block, 0, ctx->stream, p);
-        else hipLaunchKernelGGL((step_kernel<float, false>), grid, block, 0, ctx->stream, p);
-    }
+    if (ctx->storage == PDOT_STORAGE_F64) launch_step_t<double>(ctx->cost_kind, eval, grid, block, ctx->stream, p);
+    else launch_step_t<float>(ctx->cost_kind, eval, grid, block, ctx->stream, p);
     return hipGetLastError();
 }
 
@@ -610,6 +664,30 @@ void parallel_copy(void* dst, const void* src, size_t bytes) {
 
 double host_nanmax(double acc, double t) { return (acc != acc || t != t || t > acc) ? (acc != acc ? acc : t) : acc; }
 
+// cached_ot_info, src/termination.jl:94-101, from the upload pass's
+// per-block (max |C|, sum C^2) partials, in a fixed order.
+void finish_ot_info(pdot_ctx* ctx, const std::vector<double>& parts, const double* a, const double* b) {
+    double cmax = 0.0, csq = 0.0;
+    for (size_t t = 0; t < parts.size(); t += 2) {
+        cmax = host_nanmax(cmax, parts[t]);
+        csq += parts[t + 1];
+    }
+    double qmax = 0.0, qsq = 0.0;
+    for (int64_t i = 0; i < ctx->n; ++i) {
+        qmax = host_nanmax(qmax, std::fabs(a[i]));
+        qsq += a[i] * a[i];
+    }
+    for (int64_t j = 0; j < ctx->m; ++j) {
+        qmax = host_nanmax(qmax, std::fabs(b[j]));
+        qsq += b[j] * b[j];
+    }
+    ctx->ot_info[0] = cmax;
+    ctx->ot_info[1] = qmax;
+    ctx->ot_info[2] = std::sqrt(csq);
+    ctx->ot_info[3] = std::sqrt(qsq);
+    ctx->ot_ready = true;
+}
+
 }  // namespace
 
 extern "C" {
@@ -630,7 +708,6 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     const size_t mat = ctx->esz * (size_t)ctx->ld * (size_t)m;
     int rc;
-    if ((rc = dev_alloc(ctx, &ctx->C, mat))) return rc;
     for (int s = 0; s < 4; ++s) {
         if ((rc = dev_alloc(ctx, &ctx->s.X[s], mat))) return rc;
         if ((rc = dev_alloc(ctx, (void**)&ctx->s.u[s], sizeof(double) * (size_t)ctx->ld))) return rc;
@@ -639,7 +716,6 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
         if ((rc = dev_alloc(ctx, (void**)&ctx->s.c[s], sizeof(double) * (size_t)m))) return rc;
         HIPCHK(ctx, hipMemsetAsync(ctx->s.X[s], 0, mat, ctx->stream));
     }
-    HIPCHK(ctx, hipMemsetAsync(ctx->C, 0, mat, ctx->stream));
     if ((rc = dev_alloc(ctx, (void**)&ctx->a, sizeof(double) * (size_t)ctx->ld))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->b, sizeof(double) * (size_t)m))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalout, sizeof(double) * kEvalOut))) return rc;
@@ -651,10 +727,10 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     int ncu = 0, per_cu = 0;
     HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
     if (storage == PDOT_STORAGE_F64)
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false>,
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kCostDense>,
                                                                   StepShape<double>::kWaves * kWave, 0));
     else
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false>,
+        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kCostDense>,
                                                                   StepShape<float>::kWaves * kWave, 0));
     ctx->resident_blocks = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
     if ((rc = setup_geometry(ctx))) return rc;
@@ -683,6 +759,8 @@ void pdot_destroy(pdot_ctx* ctx) {
         if (ctx->pin_ev[k]) hipEventDestroy(ctx->pin_ev[k]);
     }
     if (ctx->ot_part) hipFree(ctx->ot_part);
+    if (ctx->px) hipFree(ctx->px);
+    if (ctx->py) hipFree(ctx->py);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -697,6 +775,12 @@ int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b
     const bool f32 = ctx->storage == PDOT_STORAGE_F32;
     ctx->ot_ready = false;
     int rc;
+    if (!ctx->C) {  // the cost matrix is allocated by the first dense upload
+        const size_t mat = ctx->esz * (size_t)ld * (size_t)m;
+        if ((rc = dev_alloc(ctx, &ctx->C, mat))) return rc;
+        HIPCHK(ctx, hipMemsetAsync(ctx->C, 0, mat, ctx->stream));
+    }
+    ctx->cost_kind = kCostDense;
     if ((rc = ensure_pinned(ctx))) return rc;
     if (f32 && (rc = ensure_stage(ctx))) return rc;
     const int64_t cols = f32 ? (ctx->pin_cols < ctx->stage_cols ? ctx->pin_cols : ctx->stage_cols) : ctx->pin_cols;
@@ -743,26 +827,106 @@ int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b
     HIPCHK(ctx, hipMemcpyAsync(parts.data(), ctx->ot_part, sizeof(double) * parts.size(), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
 
-    // cached_ot_info, src/termination.jl:94-101, in a fixed order
-    double cmax = 0.0, csq = 0.0;
-    for (size_t t = 0; t < parts.size(); t += 2) {
-        cmax = host_nanmax(cmax, parts[t]);
-        csq += parts[t + 1];
+    finish_ot_info(ctx, parts, a, b);
+    ctx->uploaded = true;
+    return PDOT_OK;
+}
+
+int pdot_upload_points(pdot_ctx* ctx, int32_t cost_kind, int32_t dim, const double* x, const double* y,
+                       int32_t normalize, const double* a, const double* b) {
+    if (!ctx) return PDOT_E_ARG;
+    if (!x || !y || !a || !b) return fail(ctx, PDOT_E_ARG, "pdot_upload_points: null pointer");
+    if (cost_kind != PDOT_COST_SQEUCLID && cost_kind != PDOT_COST_L1)
+        return fail(ctx, PDOT_E_ARG, "pdot_upload_points: unknown cost kind");
+    if (dim < 1 || dim > kMaxDim) return fail(ctx, PDOT_E_ARG, "pdot_upload_points: dim must be 1..3");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    const int64_t n = ctx->n, m = ctx->m, ld = ctx->ld;
+    ctx->ot_ready = false;
+    int rc;
+    if (ctx->C) {  // no dense matrix in point mode
+        hipFree(ctx->C);
+        ctx->bytes -= (int64_t)(ctx->esz * (size_t)ld * (size_t)m);
+        ctx->C = nullptr;
+        free_graphs(ctx);
     }
-    double qmax = 0.0, qsq = 0.0;
-    for (int64_t i = 0; i < n; ++i) {
-        qmax = host_nanmax(qmax, std::fabs(a[i]));
-        qsq += a[i] * a[i];
+    if (!ctx->px && (rc = dev_alloc(ctx, (void**)&ctx->px, sizeof(double) * (size_t)(kMaxDim * n)))) return rc;
+    if (!ctx->py && (rc = dev_alloc(ctx, (void**)&ctx->py, sizeof(double) * (size_t)(kMaxDim * m)))) return rc;
+    std::vector<double> xt((size_t)(dim * n));
+    for (int64_t i = 0; i < n; ++i)
+        for (int d = 0; d < dim; ++d) xt[(size_t)(d * n + i)] = x[i * dim + d];
+    HIPCHK(ctx, hipMemcpyAsync(ctx->px, xt.data(), sizeof(double) * xt.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->py, y, sizeof(double) * (size_t)(m * dim), hipMemcpyHostToDevice, ctx->stream));
+    const int cost = cost_kind == PDOT_COST_SQEUCLID ? kCostSqEuclid : kCostL1;
+    // the norms of the matrix the points define, in the chunk order of a dense upload
+    if ((rc = ensure_pinned(ctx))) return rc;
+    const int64_t cols = ctx->pin_cols;
+    const int64_t nchunks = (m + cols - 1) / cols;
+    const int64_t G = cols < kIngestBlocks ? cols : kIngestBlocks;
+    if (ctx->ot_part) {
+        hipFree(ctx->ot_part);
+        ctx->bytes -= ctx->ot_part_bytes;
+        ctx->ot_part = nullptr;
     }
-    for (int64_t j = 0; j < m; ++j) {
-        qmax = host_nanmax(qmax, std::fabs(b[j]));
-        qsq += b[j] * b[j];
+    ctx->ot_part_bytes = (int64_t)(sizeof(double) * 2 * G * nchunks);
+    if ((rc = dev_alloc(ctx, (void**)&ctx->ot_part, (size_t)ctx->ot_part_bytes))) return rc;
+    std::vector<double> parts((size_t)(2 * G * nchunks));
+    auto stats_pass = [&](double cdiv) -> int {
+        for (int64_t k = 0; k < nchunks; ++k) {
+            const int64_t j0 = k * cols;
+            const int64_t nc = (j0 + cols < m) ? cols : m - j0;
+            if (cost == kCostSqEuclid)
+                hipLaunchKernelGGL(point_stats_kernel<kCostSqEuclid>, dim3((unsigned)G), dim3(256), 0, ctx->stream, ctx->px,
+                                   ctx->py, n, (int)dim, cdiv, j0, nc, ctx->ot_part + 2 * G * k);
+            else
+                hipLaunchKernelGGL(point_stats_kernel<kCostL1>, dim3((unsigned)G), dim3(256), 0, ctx->stream, ctx->px,
+                                   ctx->py, n, (int)dim, cdiv, j0, nc, ctx->ot_part + 2 * G * k);
+            HIPCHK(ctx, hipGetLastError());
+        }
+        HIPCHK(ctx, hipMemcpyAsync(parts.data(), ctx->ot_part, sizeof(double) * parts.size(), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return PDOT_OK;
+    };
+    double cdiv = 1.0;
+    if (normalize) {  // C = raw / max(raw), as a dense caller divides by C.max()
+        if ((rc = stats_pass(1.0))) return rc;
+        double mx = 0.0;
+        for (size_t t = 0; t < parts.size(); t += 2) mx = host_nanmax(mx, parts[t]);
+        if (!(mx > 0.0)) return fail(ctx, PDOT_E_ARG, "pdot_upload_points: cannot normalise a zero cost");
+        cdiv = mx;
     }
-    ctx->ot_info[0] = cmax;
-    ctx->ot_info[1] = qmax;
-    ctx->ot_info[2] = std::sqrt(csq);
-    ctx->ot_info[3] = std::sqrt(qsq);
-    ctx->ot_ready = true;
+    if ((rc = stats_pass(cdiv))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->a, 0, sizeof(double) * ld, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->a, a, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->b, b, sizeof(double) * m, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    finish_ot_info(ctx, parts, a, b);
+    ctx->cost_kind = cost;
+    ctx->dim = dim;
+    ctx->cdiv = cdiv;
+    // the grid is sized by the occupancy of the kernel that will run
+    int ncu = 0, per_cu = 0;
+    HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    if (ctx->storage == PDOT_STORAGE_F64) {
+        if (cost == kCostSqEuclid)
+            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kCostSqEuclid>,
+                                                                      StepShape<double>::kWaves * kWave, 0));
+        else
+            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kCostL1>,
+                                                                      StepShape<double>::kWaves * kWave, 0));
+    } else {
+        if (cost == kCostSqEuclid)
+            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kCostSqEuclid>,
+                                                                      StepShape<float>::kWaves * kWave, 0));
+        else
+            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kCostL1>,
+                                                                      StepShape<float>::kWaves * kWave, 0));
+    }
+    const int32_t resident = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
+    if (resident != ctx->resident_blocks) {
+        ctx->resident_blocks = resident;
+        if ((rc = setup_geometry(ctx))) return rc;
+    }
     ctx->uploaded = true;
     return PDOT_OK;
 }

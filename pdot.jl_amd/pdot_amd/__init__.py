@@ -7,7 +7,7 @@ include/pdot.h); this package is the host control logic."""
 from __future__ import annotations
 
 from ._lib import (STORAGE_F32, STORAGE_F64, DeviceSolver, PdotError, load_library)
-from .io import OptimalTransportProblem
+from .io import OptimalTransportProblem, PointCloudProblem
 from .saddle_point import (ADAPTIVE_KKT, FIXED_FREQUENCY, KKT_GREEDY, NO_RESTART_TO_CURRENT, NO_RESTARTS,
                            RestartParameters, RestartScheme, RestartToCurrentMetric, compute_new_primal_weight,
                            construct_restart_parameters, select_initial_primal_weight)

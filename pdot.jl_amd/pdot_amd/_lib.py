@@ -96,6 +96,7 @@ SIGNATURES = {
     "pdot_last_error": (ctypes.c_char_p, [_ctx_p]),
     "pdot_upload": (ctypes.c_int, [_ctx_p, _c_double_p, _c_double_p, _c_double_p]),
     "pdot_cached_ot_info": (ctypes.c_int, [_ctx_p, _c_double_p]),
+    "pdot_upload_points": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32, _c_double_p, _c_double_p, ctypes.c_int32, _c_double_p, _c_double_p]),
     "pdot_set_start": (ctypes.c_int, [_ctx_p, _c_double_p, _c_double_p, ctypes.c_int32, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "pdot_run_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, ctypes.POINTER(SolverState)]),
     "pdot_eval": (ctypes.c_int, [_ctx_p, ctypes.POINTER(PointStats), ctypes.POINTER(PointStats), ctypes.POINTER(SolverState)]),
@@ -265,6 +266,17 @@ class DeviceSolver:
         xp = X.ctypes.data_as(_c_double_p) if X is not None else None
         self._check(self.lib.pdot_download(self.h, int(which), xp, _ptr(u), _ptr(v)), "pdot_download")
         return X, u, v
+
+    def upload_points(self, problem) -> None:
+        """Point-cloud problem (io.PointCloudProblem): the cost is formed on
+        the device each pass and never stored."""
+        from .io import COST_KINDS
+        x, y = problem.source_points, problem.target_points
+        if x.shape[0] != self.n or y.shape[0] != self.m:
+            raise ValueError(f"point clouds {x.shape[0]} x {y.shape[0]} != {(self.n, self.m)}")
+        self._check(self.lib.pdot_upload_points(self.h, COST_KINDS[problem.cost], x.shape[1], _ptr(x), _ptr(y),
+                                                int(bool(problem.normalize)), _ptr(problem.source_distribution),
+                                                _ptr(problem.target_distribution)), "pdot_upload_points")
 
     def cached_ot_info(self):
         """termination.cached_ot_info of the uploaded problem, from the device

@@ -28,7 +28,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .io import OptimalTransportProblem
+from .io import OptimalTransportProblem, PointCloudProblem
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -104,30 +104,36 @@ def l1_cost(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     return ct.T
 
 
-def uniform_sqeuclid(n: int = 1000, m: Optional[int] = None, seed: int = 1000) -> OptimalTransportProblem:
+def uniform_sqeuclid_points(n: int = 1000, m: Optional[int] = None, seed: int = 1000) -> PointCloudProblem:
     m = n if m is None else m
     g = Rng(seed)
     x = g.uniform(2 * n).reshape(n, 2)
     y = g.uniform(2 * m).reshape(m, 2)
     a = normalized(g.uniform(n))
     b = normalized(g.uniform(m))
-    return OptimalTransportProblem(sqeuclid_cost(x, y), a, b)
+    return PointCloudProblem(x, y, a, b, "sqeuclid", normalize=False)
 
 
-def gaussian_shift(n: int = 10000, m: Optional[int] = None, seed: int = 2000) -> OptimalTransportProblem:
+def uniform_sqeuclid(n: int = 1000, m: Optional[int] = None, seed: int = 1000) -> OptimalTransportProblem:
+    return uniform_sqeuclid_points(n, m, seed).dense()
+
+
+def gaussian_shift_points(n: int = 10000, m: Optional[int] = None, seed: int = 2000) -> PointCloudProblem:
     m = n if m is None else m
     g = Rng(seed)
     x = g.normal(2 * n).reshape(n, 2)
     y = g.normal(2 * m).reshape(m, 2)
     y[:, 0] += 2.0
-    C = sqeuclid_cost(x, y)
-    C /= C.max()
     a = normalized(g.uniform(n))
     b = normalized(g.uniform(m))
-    return OptimalTransportProblem(C, a, b)
+    return PointCloudProblem(x, y, a, b, "sqeuclid", normalize=True)
 
 
-def l1_mixture_3d(n: int = 30000, m: Optional[int] = None, seed: int = 3000) -> OptimalTransportProblem:
+def gaussian_shift(n: int = 10000, m: Optional[int] = None, seed: int = 2000) -> OptimalTransportProblem:
+    return gaussian_shift_points(n, m, seed).dense()
+
+
+def l1_mixture_3d_points(n: int = 30000, m: Optional[int] = None, seed: int = 3000) -> PointCloudProblem:
     m = n if m is None else m
     g = Rng(seed)
     x = g.uniform(3 * n).reshape(n, 3)
@@ -136,7 +142,11 @@ def l1_mixture_3d(n: int = 30000, m: Optional[int] = None, seed: int = 3000) -> 
     y = means[comp] + 0.1 * g.normal(3 * m).reshape(m, 3)
     a = normalized(np.exp(g.normal(n)))
     b = normalized(np.exp(g.normal(m)))
-    return OptimalTransportProblem(l1_cost(x, y), a, b)
+    return PointCloudProblem(x, y, a, b, "l1", normalize=False)
+
+
+def l1_mixture_3d(n: int = 30000, m: Optional[int] = None, seed: int = 3000) -> OptimalTransportProblem:
+    return l1_mixture_3d_points(n, m, seed).dense()
 
 
 def _gmm_points(g: Rng, k_pts: int) -> np.ndarray:

@@ -265,7 +265,9 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
     scalars in FP64).  warm_start: optional (u0, v0) initial duals; X starts at
     0 and G at (C - u0) - v0.  solver: reuse a DeviceSolver whose problem is
     already uploaded (the upload is then skipped)."""
-    nrow, ncol = original_problem.cost_matrix.shape
+    from .io import PointCloudProblem
+    points = isinstance(original_problem, PointCloudProblem)
+    nrow, ncol = original_problem.num_row, original_problem.num_col
     if params.primal_importance <= 0 or not math.isfinite(params.primal_importance):
         raise ValueError("primal_importance must be positive and finite")
     if params.termination_evaluation_frequency <= 0:
@@ -274,8 +276,11 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
     t_up = time.time()
     if solver is None:
         solver = _lib.DeviceSolver(nrow, ncol, storage=storage, device=device)
-        solver.upload(original_problem.cost_matrix, original_problem.source_distribution,
-                      original_problem.target_distribution)
+        if points:
+            solver.upload_points(original_problem)
+        else:
+            solver.upload(original_problem.cost_matrix, original_problem.source_distribution,
+                          original_problem.target_distribution)
     elif (solver.n, solver.m) != (nrow, ncol):
         raise ValueError("solver was created for another problem size")
     solver.use_graphs(use_graphs)

@@ -1,0 +1,83 @@
+"""GPU tests of the point-cloud cost form (pdot_upload_points): the cost is
+formed in registers each pass from the point clouds instead of being read
+from HBM.  The formula reproduces the dense generators' matrix bit for bit,
+so in FP64 a point-cloud solve must equal the dense solve of that matrix bit
+for bit (iterations, restarts, plan and duals); with FP32 storage (where the
+dense form stores C in FP32 and the point form keeps it exact) results are
+pinned against the HiGHS optima."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pdot_amd as P
+from pdot_amd import _lib, instances
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+FIX = json.load(open(os.path.join(HERE, "golden", "lp_optima.json")))
+POINTS = {"uniform_sqeuclid": instances.uniform_sqeuclid_points, "gaussian_shift": instances.gaussian_shift_points,
+          "l1_mixture_3d": instances.l1_mixture_3d_points}
+
+
+@pytest.mark.parametrize("make,args", [(instances.gaussian_shift_points, (700, 500, 3)),
+                                       (instances.l1_mixture_3d_points, (400, 450, 4)),
+                                       (instances.uniform_sqeuclid_points, (300, 257, 11))])
+def test_point_solve_equals_dense_solve_bitwise(make, args):
+    pts = make(*args)
+    dense = pts.dense()
+    params = P.default_parameters(1e-4)
+    a = P.optimize(params, pts)
+    b = P.optimize(params, dense)
+    assert a.termination_reason == b.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+    assert a.iteration_count == b.iteration_count
+    assert len(a.iteration_stats) == len(b.iteration_stats)
+    for sa, sb in zip(a.iteration_stats, b.iteration_stats):
+        assert sa.restart_used == sb.restart_used
+        assert sa.convergence_information[0].primal_objective == sb.convergence_information[0].primal_objective
+        assert sa.step_size == sb.step_size and sa.primal_weight == sb.primal_weight
+    assert np.array_equal(a.primal_solution, b.primal_solution)
+    assert np.array_equal(a.dual_source_solution, b.dual_source_solution)
+    assert np.array_equal(a.dual_target_solution, b.dual_target_solution)
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_point_cached_ot_info_equals_dense(storage):
+    pts = instances.gaussian_shift_points(3000, 2900, 5)  # two upload chunks
+    dense = pts.dense()
+    d1 = _lib.DeviceSolver(3000, 2900, storage=storage)
+    d1.upload_points(pts)
+    d2 = _lib.DeviceSolver(3000, 2900, storage=storage)
+    d2.upload(dense.cost_matrix, dense.source_distribution, dense.target_distribution)
+    assert d1.cached_ot_info() == d2.cached_ot_info()
+    assert d1.cached_ot_info().l_inf_norm_primal_linear_objective == 1.0  # normalised by its maximum
+    assert d1.device_bytes() < d2.device_bytes()  # no matrix allocated for C
+    d1.close()
+    d2.close()
+
+
+@pytest.mark.parametrize("storage,tol", [(_lib.STORAGE_F32, 1e-4), (_lib.STORAGE_F64, 1e-6)])
+def test_point_results_match_lp_optima(storage, tol):
+    for c in FIX["lp"]:
+        if c["generator"] not in POINTS or c["n"] * c["m"] > 1_000_000:
+            continue
+        pts = POINTS[c["generator"]](c["n"], c["m"], seed=c["seed"])
+        out = P.optimize(P.default_parameters(tol), pts, storage=storage)
+        assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL, c
+        pobj = float(np.sum(pts.dense().cost_matrix * out.primal_solution))
+        assert abs(pobj - c["optimum"]) <= 10 * tol * (1 + abs(c["optimum"])), c
+
+
+def test_point_first_ten_evaluations_match_oracle():
+    pts = instances.l1_mixture_3d_points(333, 270, 9)
+    params = P.default_parameters(1e-4, iteration_limit=10)
+    out = P.optimize(params, pts)
+    ref = O.optimize(params, pts.dense(), mode=O.FUSED)
+    assert out.iteration_count == ref["iteration_count"]
+    for s_dev, s_ref in zip(out.iteration_stats, ref["stats"]):
+        ci = s_dev.convergence_information[0]
+        for f in ("primal_objective", "dual_objective", "l2_primal_residual", "l2_dual_residual"):
+            np.testing.assert_allclose(getattr(ci, f), s_ref[f], rtol=1e-9, atol=1e-15, err_msg=f)
