@@ -48,15 +48,43 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 #ifndef PDOT_F32_PIPE
 #define PDOT_F32_PIPE 1
 #endif
-template <typename T>
+#ifndef PDOT_F64P_WAVES
+#define PDOT_F64P_WAVES PDOT_F64_WAVES
+#endif
+#ifndef PDOT_F64P_UNROLL
+#define PDOT_F64P_UNROLL PDOT_F64_UNROLL
+#endif
+#ifndef PDOT_F64P_PIPE
+#define PDOT_F64P_PIPE PDOT_F64_PIPE
+#endif
+#ifndef PDOT_F32P_WAVES
+#define PDOT_F32P_WAVES PDOT_F32_WAVES
+#endif
+#ifndef PDOT_F32P_UNROLL
+#define PDOT_F32P_UNROLL PDOT_F32_UNROLL
+#endif
+#ifndef PDOT_F32P_PIPE
+#define PDOT_F32P_PIPE PDOT_F32_PIPE
+#endif
+// POINTS: the cost is formed from point clouds (more registers per lane, no
+// C stream), which has its own best shape.
+template <typename T, bool POINTS = false>
 struct StepShape;
 template <>
-struct StepShape<double> {
+struct StepShape<double, false> {
     static constexpr int kWaves = PDOT_F64_WAVES, kUnroll = PDOT_F64_UNROLL, kPipe = PDOT_F64_PIPE;
 };
 template <>
-struct StepShape<float> {
+struct StepShape<float, false> {
     static constexpr int kWaves = PDOT_F32_WAVES, kUnroll = PDOT_F32_UNROLL, kPipe = PDOT_F32_PIPE;
+};
+template <>
+struct StepShape<double, true> {
+    static constexpr int kWaves = PDOT_F64P_WAVES, kUnroll = PDOT_F64P_UNROLL, kPipe = PDOT_F64P_PIPE;
+};
+template <>
+struct StepShape<float, true> {
+    static constexpr int kWaves = PDOT_F32P_WAVES, kUnroll = PDOT_F32P_UNROLL, kPipe = PDOT_F32P_PIPE;
 };
 
 // ------------------------------------------------------------------ helpers

@@ -93,8 +93,9 @@ struct ReduceParams {
 // [rb*TR, (rb+1)*TR) of the columns cb, cb+Cb, ... (interleaved) or of the
 // contiguous range [cb*TC, (cb+1)*TC); step_tile does the work.
 template <typename T, bool EVAL, int COST>
-__global__ __launch_bounds__(StepShape<T>::kWaves * kWave) void step_kernel(StepParams p) {
-    constexpr int WAVES = StepShape<T>::kWaves;
+__global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave)) void step_kernel(StepParams p) {
+    using Shape = StepShape<T, COST != kCostDense>;
+    constexpr int WAVES = Shape::kWaves;
     constexpr int TR = WAVES * kWave * VecT<T>::N;
     __shared__ pdot_solver_state sst;
     __shared__ double red[WAVES * kWave];
@@ -146,7 +147,7 @@ __global__ __launch_bounds__(StepShape<T>::kWaves * kWave) void step_kernel(Step
     a.gzero = sst.gzero != 0;
     a.do_step = do_step;
     double dd = 0.0, ev[kMatStats];
-    step_tile<T, EVAL, StepShape<T>::kUnroll, WAVES, StepShape<T>::kPipe, COST>(a, colbuf, red, &dd, ev);
+    step_tile<T, EVAL, Shape::kUnroll, WAVES, Shape::kPipe, COST>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
         if (do_step) p.sqpart[blockIdx.x] = dd;
         if (EVAL) {
@@ -426,12 +427,18 @@ int fail(pdot_ctx* ctx, int code, const std::string& msg) {
         }                                                                                              \
     } while (0)
 
-int step_waves(int32_t storage) {
-    return storage == PDOT_STORAGE_F64 ? StepShape<double>::kWaves : StepShape<float>::kWaves;
+int step_waves(int32_t storage, int cost) {
+    const bool pts = cost != kCostDense;
+    if (storage == PDOT_STORAGE_F64) return pts ? StepShape<double, true>::kWaves : StepShape<double, false>::kWaves;
+    return pts ? StepShape<float, true>::kWaves : StepShape<float, false>::kWaves;
 }
-int rows_per_block(int32_t storage) { return step_waves(storage) * kWave * (storage == PDOT_STORAGE_F64 ? 2 : 4); }
-static_assert(StepShape<double>::kWaves >= 1 && StepShape<double>::kWaves <= 16, "step kernel waves");
-static_assert(StepShape<float>::kWaves >= 1 && StepShape<float>::kWaves <= 16, "step kernel waves");
+int rows_per_block(int32_t storage, int cost) {
+    return step_waves(storage, cost) * kWave * (storage == PDOT_STORAGE_F64 ? 2 : 4);
+}
+static_assert(StepShape<double, false>::kWaves >= 1 && StepShape<double, false>::kWaves <= 16, "step kernel waves");
+static_assert(StepShape<float, false>::kWaves >= 1 && StepShape<float, false>::kWaves <= 16, "step kernel waves");
+static_assert(StepShape<double, true>::kWaves >= 1 && StepShape<double, true>::kWaves <= 16, "step kernel waves");
+static_assert(StepShape<float, true>::kWaves >= 1 && StepShape<float, true>::kWaves <= 16, "step kernel waves");
 
 int dev_alloc(pdot_ctx* ctx, void** p, size_t bytes) {
     if (bytes == 0) bytes = 8;
@@ -461,7 +468,7 @@ void free_partials(pdot_ctx* ctx) {
 int setup_geometry(pdot_ctx* ctx) {
     free_graphs(ctx);
     free_partials(ctx);
-    const int TR = rows_per_block(ctx->storage);
+    const int TR = rows_per_block(ctx->storage, ctx->cost_kind);
     const int64_t R = (ctx->n + TR - 1) / TR;
     int64_t Cb;
     if (ctx->cb_hint > 0) {
@@ -558,7 +565,7 @@ void launch_step_t(int cost, bool eval, dim3 grid, dim3 block, hipStream_t strea
 
 hipError_t launch_step(pdot_ctx* ctx, bool eval) {
     StepParams p = step_params(ctx);
-    dim3 grid(ctx->nstep), block(step_waves(ctx->storage) * kWave);
+    dim3 grid(ctx->nstep), block(step_waves(ctx->storage, ctx->cost_kind) * kWave);
     if (ctx->storage == PDOT_STORAGE_F64) launch_step_t<double>(ctx->cost_kind, eval, grid, block, ctx->stream, p);
     else launch_step_t<float>(ctx->cost_kind, eval, grid, block, ctx->stream, p);
     return hipGetLastError();
@@ -688,6 +695,27 @@ void finish_ot_info(pdot_ctx* ctx, const std::vector<double>& parts, const doubl
     ctx->ot_ready = true;
 }
 
+template <typename T, int COST>
+hipError_t step_occupancy(int* per_cu) {
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, step_kernel<T, false, COST>,
+                                                        StepShape<T, COST != kCostDense>::kWaves * kWave, 0);
+}
+
+// Resident step blocks for the kernel the context will run (storage and cost
+// form) and the launch geometry derived from it.
+int configure_kernel(pdot_ctx* ctx) {
+    int ncu = 0, per_cu = 0;
+    HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const bool f64 = ctx->storage == PDOT_STORAGE_F64;
+    hipError_t e;
+    if (ctx->cost_kind == kCostSqEuclid) e = f64 ? step_occupancy<double, kCostSqEuclid>(&per_cu) : step_occupancy<float, kCostSqEuclid>(&per_cu);
+    else if (ctx->cost_kind == kCostL1) e = f64 ? step_occupancy<double, kCostL1>(&per_cu) : step_occupancy<float, kCostL1>(&per_cu);
+    else e = f64 ? step_occupancy<double, kCostDense>(&per_cu) : step_occupancy<float, kCostDense>(&per_cu);
+    HIPCHK(ctx, e);
+    ctx->resident_blocks = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
+    return setup_geometry(ctx);
+}
+
 }  // namespace
 
 extern "C" {
@@ -724,16 +752,7 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     HIPCHK(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(unsigned int), ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_state, sizeof(pdot_solver_state), hipHostMallocDefault));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_eval, sizeof(double) * kEvalOut, hipHostMallocDefault));
-    int ncu = 0, per_cu = 0;
-    HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device));
-    if (storage == PDOT_STORAGE_F64)
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kCostDense>,
-                                                                  StepShape<double>::kWaves * kWave, 0));
-    else
-        HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kCostDense>,
-                                                                  StepShape<float>::kWaves * kWave, 0));
-    ctx->resident_blocks = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
-    if ((rc = setup_geometry(ctx))) return rc;
+    if ((rc = configure_kernel(ctx))) return rc;
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return PDOT_OK;
 }
@@ -780,7 +799,10 @@ int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b
         if ((rc = dev_alloc(ctx, &ctx->C, mat))) return rc;
         HIPCHK(ctx, hipMemsetAsync(ctx->C, 0, mat, ctx->stream));
     }
-    ctx->cost_kind = kCostDense;
+    if (ctx->cost_kind != kCostDense) {
+        ctx->cost_kind = kCostDense;
+        if ((rc = configure_kernel(ctx))) return rc;
+    }
     if ((rc = ensure_pinned(ctx))) return rc;
     if (f32 && (rc = ensure_stage(ctx))) return rc;
     const int64_t cols = f32 ? (ctx->pin_cols < ctx->stage_cols ? ctx->pin_cols : ctx->stage_cols) : ctx->pin_cols;
@@ -904,29 +926,7 @@ int pdot_upload_points(pdot_ctx* ctx, int32_t cost_kind, int32_t dim, const doub
     ctx->cost_kind = cost;
     ctx->dim = dim;
     ctx->cdiv = cdiv;
-    // the grid is sized by the occupancy of the kernel that will run
-    int ncu = 0, per_cu = 0;
-    HIPCHK(ctx, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    if (ctx->storage == PDOT_STORAGE_F64) {
-        if (cost == kCostSqEuclid)
-            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kCostSqEuclid>,
-                                                                      StepShape<double>::kWaves * kWave, 0));
-        else
-            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<double, false, kCostL1>,
-                                                                      StepShape<double>::kWaves * kWave, 0));
-    } else {
-        if (cost == kCostSqEuclid)
-            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kCostSqEuclid>,
-                                                                      StepShape<float>::kWaves * kWave, 0));
-        else
-            HIPCHK(ctx, hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, step_kernel<float, false, kCostL1>,
-                                                                      StepShape<float>::kWaves * kWave, 0));
-    }
-    const int32_t resident = (per_cu > 0 ? per_cu : 1) * (ncu > 0 ? ncu : 1);
-    if (resident != ctx->resident_blocks) {
-        ctx->resident_blocks = resident;
-        if ((rc = setup_geometry(ctx))) return rc;
-    }
+    if ((rc = configure_kernel(ctx))) return rc;  // the point kernel's shape and occupancy
     ctx->uploaded = true;
     return PDOT_OK;
 }
@@ -1219,7 +1219,7 @@ int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint, int32_t interleave) 
 
 int pdot_get_launch(pdot_ctx* ctx, int32_t* rows_per_blk, int32_t* col_blocks, int32_t* cols_per_block, int32_t* blocks) {
     if (!ctx) return PDOT_E_ARG;
-    if (rows_per_blk) *rows_per_blk = rows_per_block(ctx->storage);
+    if (rows_per_blk) *rows_per_blk = rows_per_block(ctx->storage, ctx->cost_kind);
     if (col_blocks) *col_blocks = ctx->Cb;
     if (cols_per_block) *cols_per_block = ctx->TC;
     if (blocks) *blocks = ctx->nstep;

@@ -290,3 +290,18 @@ def test_policy_variants_first_evaluations_match_oracle(kind):
         ci = s_dev.convergence_information[0]
         for f in EVAL_FIELDS:
             np.testing.assert_allclose(getattr(ci, f), s_ref[f], rtol=1e-9, atol=1e-15, err_msg=f)
+
+
+def test_fp32_storage_objective_within_1e_6_of_tight_fp64():
+    """The FP32-storage form's objective error from the storage itself: solved
+    to 1e-7 it lies within 1e-6 relative of the FP64 form solved to 1e-9 (the
+    tightened reference solve).  At looser tolerances the difference is the
+    tolerance's, not the storage's."""
+    prob = instances.gaussian_shift(300, 280, seed=77)
+    tight = P.optimize(P.default_parameters(1e-9, iteration_limit=400000, record_iteration_stats=False), prob)
+    f32 = P.optimize(P.default_parameters(1e-7, iteration_limit=400000, record_iteration_stats=False), prob,
+                     storage=_lib.STORAGE_F32)
+    assert f32.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+    o64 = tight.iteration_stats[-1].convergence_information[0].primal_objective
+    o32 = f32.iteration_stats[-1].convergence_information[0].primal_objective
+    assert abs(o32 - o64) / abs(o64) < 1e-6, (o32, o64)
