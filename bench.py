@@ -244,7 +244,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=10000)
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-iters", type=int, default=12)
+    ap.add_argument("--cpu-iters", type=int, default=80, help="iteration limit of the CPU baseline sample (about 15 s)")
     ap.add_argument("--instances", type=int, default=4, help="held-out instances of the gmm_warm_start workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")

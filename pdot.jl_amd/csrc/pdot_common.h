@@ -58,13 +58,13 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 #define PDOT_F64P_PIPE PDOT_F64_PIPE
 #endif
 #ifndef PDOT_F32P_WAVES
-#define PDOT_F32P_WAVES PDOT_F32_WAVES
+#define PDOT_F32P_WAVES 4
 #endif
 #ifndef PDOT_F32P_UNROLL
-#define PDOT_F32P_UNROLL PDOT_F32_UNROLL
+#define PDOT_F32P_UNROLL 2
 #endif
 #ifndef PDOT_F32P_PIPE
-#define PDOT_F32P_PIPE PDOT_F32_PIPE
+#define PDOT_F32P_PIPE 0  // the cost's registers leave no room for a prefetched batch
 #endif
 // POINTS: the cost is formed from point clouds (more registers per lane, no
 // C stream), which has its own best shape.

@@ -118,7 +118,7 @@ extern "C" int pdot_write_julia_vector(const char* path, const double* x, int64_
         return PDOT_E_ARG;
     }
     // Chunks are formatted in parallel and written in order.
-    const int64_t chunk = 1 << 20;
+    const int64_t chunk = 1 << 18;  // values per thread and round (about 8.6 MB of text)
     unsigned hw = std::thread::hardware_concurrency();
     const int nthreads = (int)std::max(1u, std::min(hw == 0 ? 1u : hw, 16u));
     std::vector<std::vector<char>> bufs(nthreads, std::vector<char>((size_t)chunk * 33));
