@@ -44,6 +44,7 @@ constexpr int kBBlock = kWave * kBWaves;
 constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block
 constexpr int kBRedBlock = kWave * kBRedWaves;
 constexpr int kMinColTiles = 32;           // column tiles per instance (at least)
+constexpr int kHeavyParts = 32;            // chunks with more partials are reduced by a whole block
 constexpr int kMinTileCols = 4;            // columns per tile (at least)
 
 struct BInst {
@@ -75,7 +76,7 @@ struct BParams {
     unsigned int* counter;
     const BTile* tiles;
     const BChunk* chunks;
-    const int32_t* nwork;  // {ntiles, nchunks}
+    const int32_t* nwork;  // {ntiles, heavy chunks, light chunks}
     double* evalout;       // kEvalOut per instance
 };
 
@@ -165,8 +166,9 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
     __shared__ int amlast;
     __shared__ double psum[kBRedWaves][kWave];
     const int tid = threadIdx.x;
-    const int nchunks = p.nwork[1];
-    for (int t = blockIdx.x; t < nchunks; t += gridDim.x) {
+    const int nheavy = p.nwork[1], nlight = p.nwork[2];
+    // chunks with many partials: the block's waves split the partials
+    for (int t = blockIdx.x; t < nheavy; t += gridDim.x) {
         if (tid == 0) {
             sch = p.chunks[t];
             sinst = p.inst[sch.inst];
@@ -236,6 +238,77 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
             }
         }
         __syncthreads();
+    }
+    // the other chunks: one wave each, no barriers
+    __shared__ pdot_solver_state wst[kBRedWaves];
+    const int w = tid / kWave, lane = tid & (kWave - 1);
+    for (int item = blockIdx.x * kBRedWaves + w; item < nlight; item += gridDim.x * kBRedWaves) {
+        const BChunk sch = p.chunks[nheavy + item];
+        const BInst& in = p.inst[sch.inst];
+        const pdot_solver_state& gs = p.st[sch.inst];
+        const bool do_step = !gs.numerical_error && (EVAL || gs.step_budget > 0);
+        if (!do_step && !EVAL) continue;
+        const bool rows = sch.rows != 0;
+        const bool avg_empty = gs.avg_count == 0;
+        const int cs = gs.cur_slot, as = avg_empty ? cs : gs.avg_slot;
+        const int64_t* yo = rows ? in.u : in.v;
+        const int64_t* so = rows ? in.r : in.c;
+        ChunkArgs c;
+        c.rows = rows;
+        c.k0 = (int64_t)sch.chunk * kWave;
+        c.len = rows ? in.n : in.m;
+        c.part = p.part + (rows ? in.rowpart : in.colpart);
+        c.pstride = rows ? in.ld : in.m;
+        c.np = rows ? in.Cb : in.R;
+        c.q = p.vec + (rows ? in.a : in.b);
+        c.y_cur = p.vec + yo[cs];
+        c.s_cur = p.vec + so[cs];
+        c.y_avg = p.vec + yo[as];
+        c.s_avg = p.vec + so[as];
+        c.y_new = p.vec + yo[gs.spare0];
+        c.s_new = p.vec + so[gs.spare0];
+        c.y_anew = p.vec + yo[gs.spare1];
+        c.s_anew = p.vec + so[gs.spare1];
+        const int64_t lc = rows ? sch.chunk : in.NBr + sch.chunk;  // chunk index within the instance
+        c.sqpart = p.part + in.sqpart;
+        c.sq_lo = lc * in.sqchunk;
+        c.sq_hi = c.sq_lo + in.sqchunk < in.nstep ? c.sq_lo + in.sqchunk : in.nstep;
+        c.eta = gs.step_size;
+        c.omega = gs.primal_weight;
+        avg_weights(gs, &c.nw, &c.aw);
+        c.avg_empty = avg_empty;
+        c.do_step = do_step;
+        double out3[3], ev[kVecStats];
+        reduce_chunk_wave<EVAL>(c, out3, ev);
+        int last = 0;
+        if (lane == 0) {
+            if (EVAL) {
+                for (int q = 0; q < kVecStats; ++q) p.red[in.evalvec + lc * kVecStats + q] = ev[q];
+            }
+            if (do_step) {
+                double* rp = p.red + in.redpart + 3 * lc;
+                rp[0] = out3[0];
+                rp[1] = out3[1];
+                rp[2] = out3[2];
+                __threadfence();
+                const unsigned prev = atomicAdd(p.counter + sch.inst, 1u);
+                last = (prev == (unsigned)in.nred - 1) ? 1 : 0;
+            }
+        }
+        last = __shfl(last, 0);
+        if (last) {
+            // every chunk of the instance has finished: commit its pass
+            __threadfence();
+            if (lane == 0) wst[w] = p.st[sch.inst];
+            __builtin_amdgcn_wave_barrier();
+            if (!EVAL) reduce_and_commit(&wst[w], p.red + in.redpart, in.nred);
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) {
+                if (EVAL) wst[w].pending = 1;
+                p.st[sch.inst] = wst[w];
+                p.counter[sch.inst] = 0u;
+            }
+        }
     }
 }
 
@@ -329,7 +402,7 @@ struct pdot_batch {
     double* h_eval = nullptr;           // pinned, count * kEvalOut
     std::vector<int32_t> active;
     int64_t max_tiles = 0, max_chunks = 0, tile_cap = 0;
-    int32_t ntiles = 0, nchunks = 0;
+    int32_t ntiles = 0, nchunks = 0, nheavy = 0;
     int32_t cols_per_tile = 64;
     int32_t step_grid = 0, reduce_grid = 0, grid_mult = 1;
     int32_t step_resident = 0, reduce_resident = 0;
@@ -443,7 +516,7 @@ int64_t layout_partials(pdot_batch* b) {
 // Rebuild the tile and chunk lists from the active set (largest tiles first).
 int rebuild_work(pdot_batch* b) {
     std::vector<BTile> tiles;
-    std::vector<BChunk> chunks;
+    std::vector<BChunk> chunks, heavy, light;
     const int TR = rows_per_tile(b->storage);
     std::vector<std::pair<int64_t, BTile>> weighted;
     for (int32_t k = 0; k < b->count; ++k) {
@@ -456,16 +529,21 @@ int rebuild_work(pdot_batch* b) {
                 weighted.push_back({rows * cols, BTile{k, rb, cb, 0}});
             }
         }
-        for (int32_t c = 0; c < in.NBr; ++c) chunks.push_back(BChunk{k, 1, c, 0});
-        for (int32_t c = 0; c < in.NBc; ++c) chunks.push_back(BChunk{k, 0, c, 0});
+        // row chunks sum Cb partials, column chunks R
+        for (int32_t c = 0; c < in.NBr; ++c) (in.Cb > kHeavyParts ? heavy : light).push_back(BChunk{k, 1, c, 0});
+        for (int32_t c = 0; c < in.NBc; ++c) (in.R > kHeavyParts ? heavy : light).push_back(BChunk{k, 0, c, 0});
     }
+    chunks.reserve(heavy.size() + light.size());
+    chunks.insert(chunks.end(), heavy.begin(), heavy.end());
+    chunks.insert(chunks.end(), light.begin(), light.end());
     std::stable_sort(weighted.begin(), weighted.end(),
                      [](const std::pair<int64_t, BTile>& x, const std::pair<int64_t, BTile>& y) { return x.first > y.first; });
     tiles.reserve(weighted.size());
     for (auto& w : weighted) tiles.push_back(w.second);
     b->ntiles = (int32_t)tiles.size();
     b->nchunks = (int32_t)chunks.size();
-    int32_t nw[2] = {b->ntiles, b->nchunks};
+    b->nheavy = (int32_t)heavy.size();
+    int32_t nw[3] = {b->ntiles, b->nheavy, (int32_t)light.size()};
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     if (!tiles.empty())
         BHIPCHK(b, hipMemcpy(b->d_tiles, tiles.data(), sizeof(BTile) * tiles.size(), hipMemcpyHostToDevice));
@@ -507,7 +585,8 @@ int setup_batch_partials(pdot_batch* b) {
     // persistent grids sized to the work: no idle blocks are launched once
     // few instances remain
     const int32_t sg = (int32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)b->step_resident * b->grid_mult, b->ntiles));
-    const int32_t rg = std::max(1, std::min(b->reduce_resident, b->nchunks));
+    const int32_t reduce_items = b->nheavy + (b->nchunks - b->nheavy + kBRedWaves - 1) / kBRedWaves;
+    const int32_t rg = std::max(1, std::min(b->reduce_resident, reduce_items));
     if (sg != b->step_grid || rg != b->reduce_grid) bfree_graphs(b);  // captured launches hold the grid
     b->step_grid = sg;
     b->reduce_grid = rg;
@@ -630,7 +709,7 @@ int pdot_batch_create(pdot_batch** out, int32_t count, const int64_t* n, const i
     if ((rc = balloc(b, (void**)&b->d_inst, sizeof(BInst) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_st, sizeof(pdot_solver_state) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_counter, sizeof(unsigned int) * count))) return rc;
-    if ((rc = balloc(b, (void**)&b->d_nwork, sizeof(int32_t) * 2))) return rc;
+    if ((rc = balloc(b, (void**)&b->d_nwork, sizeof(int32_t) * 3))) return rc;
     if ((rc = balloc(b, (void**)&b->d_active, sizeof(int32_t) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_evalout, sizeof(double) * kEvalOut * count))) return rc;
     const int64_t nred_total = layout_reductions(b);

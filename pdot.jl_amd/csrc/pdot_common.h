@@ -269,9 +269,9 @@ __device__ inline void controller_commit(pdot_solver_state& st, double dot, doub
 // chunk) in a fixed order and commit.  Called by the 64 lanes of one wave;
 // lane 0 updates *sst.
 __device__ inline void reduce_and_commit(pdot_solver_state* sst, const double* redpart, int nred) {
-    const int tid = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int k = tid; k < nred; k += kWave) {
+    for (int k = lane; k < nred; k += kWave) {
         d0 += redpart[3 * k + 0];
         d1 += redpart[3 * k + 1];
         d2 += redpart[3 * k + 2];
@@ -279,7 +279,7 @@ __device__ inline void reduce_and_commit(pdot_solver_state* sst, const double* r
     d0 = wave_sum(d0);
     d1 = wave_sum(d1);
     d2 = wave_sum(d2);
-    if (tid == 0) controller_commit(*sst, d0, d1, d2);
+    if (lane == 0) controller_commit(*sst, d0, d1, d2);
 }
 
 // ---------------------------------------------------------------- vectors
@@ -622,102 +622,141 @@ struct ChunkArgs {
 // EVAL, also the vector parts of the evaluation of the input point and its
 // average.  Called by all threads; thread 0 receives out3 = {dot, yy, dx}
 // and ev_out[kVecStats].
+// The chunk's work after the partial sums (one wave; `sum` is this lane's
+// total of the partials): the dual update, the averages, the chunk's parts
+// of dr'du + dc'dv, ||dy||^2 and ||Delta||^2, and with EVAL the vector parts
+// of the evaluation.  Lane 0 receives out3 and ev_out.
+template <bool EVAL>
+__device__ __forceinline__ void finish_chunk(const ChunkArgs& c, int lane, double sum, double* out3, double* ev_out) {
+    const int64_t k = c.k0 + lane;
+    const bool do_step = c.do_step;
+    const bool avg_empty = c.avg_empty;
+    const double eta = c.eta, omega = c.omega, nw = c.nw, aw = c.aw;
+    double dot = 0.0, yy = 0.0;
+    double ev[kVecStats];
+#pragma unroll
+    for (int t = 0; t < kVecStats; ++t) ev[t] = 0.0;
+    if (k < c.len) {
+        const bool rows = c.rows;
+        const double q = c.q[k];
+        const double s_k = c.s_cur[k];
+        const double y_k = c.y_cur[k];
+        if (do_step) {
+            const double sn = sum + s_k;
+            const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
+            const double ds = sn - s_k, dy = yn - y_k;
+            dot = ds * dy;
+            yy = dy * dy;
+            c.s_new[k] = sn;
+            c.y_new[k] = yn;
+            if (!avg_empty) {
+                const double t1 = c.s_avg[k] + sn * nw;
+                c.s_anew[k] = t1 * aw;
+                const double t2 = c.y_avg[k] + yn * nw;
+                c.y_anew[k] = t2 * aw;
+            }
+        }
+        if (EVAL) {
+            const double t = q - s_k;
+            ev[0] = fabs(t);
+            ev[1] = t * t;
+            ev[2] = q * y_k;
+            ev[3] = rows ? fabs(y_k) : 0.0;
+            ev[4] = rows ? y_k * y_k : 0.0;
+            ev[5] = y_k * y_k;
+            if (!avg_empty) {
+                const double sa = c.s_avg[k], ya = c.y_avg[k];
+                const double ta = q - sa;
+                ev[6] = fabs(ta);
+                ev[7] = ta * ta;
+                ev[8] = q * ya;
+                ev[9] = rows ? fabs(ya) : 0.0;
+                ev[10] = rows ? ya * ya : 0.0;
+                ev[11] = ya * ya;
+            }
+        }
+    }
+    if (EVAL) {
+#pragma unroll
+        for (int t = 0; t < kVecStats; ++t) {
+            const bool is_max = (t == 0 || t == 3 || t == 6 || t == 9);
+            const double v = is_max ? wave_max(ev[t]) : wave_sum(ev[t]);
+            if (lane == 0) ev_out[t] = v;
+        }
+    }
+    if (do_step) {
+        double dx = 0.0;
+        for (int64_t t = c.sq_lo + lane; t < c.sq_hi; t += kWave) dx += c.sqpart[t];
+        dot = wave_sum(dot);
+        yy = wave_sum(yy);
+        dx = wave_sum(dx);
+        if (lane == 0) {
+            out3[0] = dot;
+            out3[1] = yy;
+            out3[2] = dx;
+        }
+    }
+}
+
+// Sum of partials t0 <= t < t1 at position k, with eight independent
+// accumulators (eight loads in flight per lane) combined in a fixed order.
+__device__ __forceinline__ double sum_partials(const ChunkArgs& c, int64_t k, int t0, int t1) {
+    constexpr int A = 8;
+    double acc[A];
+#pragma unroll
+    for (int q = 0; q < A; ++q) acc[q] = 0.0;
+    int t = t0;
+    for (; t + A <= t1; t += A) {
+#pragma unroll
+        for (int q = 0; q < A; ++q) acc[q] += c.part[(int64_t)(t + q) * c.pstride + k];
+    }
+    for (; t < t1; ++t) acc[0] += c.part[(int64_t)t * c.pstride + k];
+    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+// Finishes one pass for a chunk (src/solver_gpu.jl:172-173, 177-234,
+// 267-288; src/saddle_point_gpu.jl:140-221): r+ = rowsum(Delta) + r,
+// u+ = u + eta*omega*(a - 2 r+ + r), the averages of u, v, r, c, and the
+// chunk's parts of dr'du + dc'dv, ||dy||^2 and ||Delta||^2.  The partials are
+// split into RWAVES contiguous ranges (one per wave), then combined in a fixed
+// order by wave 0, which finishes the chunk.  Called by all threads of the
+// block; thread 0 receives out3 = {dot, yy, dx} and ev_out[kVecStats].
 template <bool EVAL, int RWAVES>
 __device__ __forceinline__ void reduce_chunk(const ChunkArgs& c, double (*psum)[kWave], double* out3, double* ev_out) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
     const int w = tid / kWave;
     const int64_t k = c.k0 + lane;
-    const bool do_step = c.do_step;
-    if (do_step) {
-        constexpr int A = 8;  // independent accumulators: A loads in flight per lane
-        double acc[A];
-#pragma unroll
-        for (int q = 0; q < A; ++q) acc[q] = 0.0;
+    if (c.do_step) {
+        double s = 0.0;
         if (k < c.len) {
             const int t0 = (int)(((int64_t)c.np * w) / RWAVES);
             const int t1 = (int)(((int64_t)c.np * (w + 1)) / RWAVES);
-            int t = t0;
-            for (; t + A <= t1; t += A) {
-#pragma unroll
-                for (int q = 0; q < A; ++q) acc[q] += c.part[(int64_t)(t + q) * c.pstride + k];
-            }
-            for (; t < t1; ++t) acc[0] += c.part[(int64_t)t * c.pstride + k];
+            s = sum_partials(c, k, t0, t1);
         }
-        psum[w][lane] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+        psum[w][lane] = s;
     }
     __syncthreads();
     if (w == 0) {
-        const bool avg_empty = c.avg_empty;
-        const double eta = c.eta, omega = c.omega, nw = c.nw, aw = c.aw;
-        double dot = 0.0, yy = 0.0;
-        double ev[kVecStats];
+        double sum = 0.0;
+        if (c.do_step) {
+            sum = psum[0][lane];
 #pragma unroll
-        for (int t = 0; t < kVecStats; ++t) ev[t] = 0.0;
-        if (k < c.len) {
-            const bool rows = c.rows;
-            const double q = c.q[k];
-            const double s_k = c.s_cur[k];
-            const double y_k = c.y_cur[k];
-            if (do_step) {
-                double sum = psum[0][lane];
-#pragma unroll
-                for (int t = 1; t < RWAVES; ++t) sum += psum[t][lane];
-                const double sn = sum + s_k;
-                const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
-                const double ds = sn - s_k, dy = yn - y_k;
-                dot = ds * dy;
-                yy = dy * dy;
-                c.s_new[k] = sn;
-                c.y_new[k] = yn;
-                if (!avg_empty) {
-                    const double t1 = c.s_avg[k] + sn * nw;
-                    c.s_anew[k] = t1 * aw;
-                    const double t2 = c.y_avg[k] + yn * nw;
-                    c.y_anew[k] = t2 * aw;
-                }
-            }
-            if (EVAL) {
-                const double t = q - s_k;
-                ev[0] = fabs(t);
-                ev[1] = t * t;
-                ev[2] = q * y_k;
-                ev[3] = rows ? fabs(y_k) : 0.0;
-                ev[4] = rows ? y_k * y_k : 0.0;
-                ev[5] = y_k * y_k;
-                if (!avg_empty) {
-                    const double sa = c.s_avg[k], ya = c.y_avg[k];
-                    const double ta = q - sa;
-                    ev[6] = fabs(ta);
-                    ev[7] = ta * ta;
-                    ev[8] = q * ya;
-                    ev[9] = rows ? fabs(ya) : 0.0;
-                    ev[10] = rows ? ya * ya : 0.0;
-                    ev[11] = ya * ya;
-                }
-            }
+            for (int t = 1; t < RWAVES; ++t) sum += psum[t][lane];
         }
-        if (EVAL) {
-#pragma unroll
-            for (int t = 0; t < kVecStats; ++t) {
-                const bool is_max = (t == 0 || t == 3 || t == 6 || t == 9);
-                const double v = is_max ? wave_max(ev[t]) : wave_sum(ev[t]);
-                if (lane == 0) ev_out[t] = v;
-            }
-        }
-        if (do_step) {
-            double dx = 0.0;
-            for (int64_t t = c.sq_lo + lane; t < c.sq_hi; t += kWave) dx += c.sqpart[t];
-            dot = wave_sum(dot);
-            yy = wave_sum(yy);
-            dx = wave_sum(dx);
-            if (lane == 0) {
-                out3[0] = dot;
-                out3[1] = yy;
-                out3[2] = dx;
-            }
-        }
+        finish_chunk<EVAL>(c, lane, sum, out3, ev_out);
     }
+}
+
+// The same for one wave alone (no barrier): the wave sums all partials of its
+// chunk itself.  For chunks with few partials, so that the waves of a block
+// finish independent chunks in parallel.
+template <bool EVAL>
+__device__ __forceinline__ void reduce_chunk_wave(const ChunkArgs& c, double* out3, double* ev_out) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t k = c.k0 + lane;
+    const double sum = (c.do_step && k < c.len) ? sum_partials(c, k, 0, c.np) : 0.0;
+    finish_chunk<EVAL>(c, lane, sum, out3, ev_out);
 }
 
 }  // namespace
