@@ -132,6 +132,7 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             a.n = sinst.n;
             a.ld = sinst.ld;
             a.row0 = (int64_t)stl.rb * TR;
+            a.row_end = (a.row0 + TR < sinst.n) ? a.row0 + TR : sinst.n;
             a.j0 = (int64_t)stl.cb * sinst.TC;
             a.jstep = 1;
             a.ncols = ((a.j0 + sinst.TC < sinst.m) ? a.j0 + sinst.TC : sinst.m) - a.j0;

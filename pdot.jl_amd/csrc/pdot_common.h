@@ -364,7 +364,8 @@ __device__ __forceinline__ double point_cost(const double* xi, const double* yj,
 }
 
 // Everything one block needs to advance one tile of one problem by a PDHG
-// pass: rows [row0, row0 + WAVES*64*VEC) of the columns j0 + t*jstep,
+// pass: rows [row0, row_end) (at most WAVES*64*VEC rows; row_end - row0 a
+// multiple of VEC unless row_end = n) of the columns j0 + t*jstep,
 // t < ncols.
 template <typename T>
 struct TileArgs {
@@ -383,7 +384,7 @@ struct TileArgs {
     const double* va;
     double* rowpart;  // partial row sums of this tile, indexed by row
     double* colpart;  // partial column sums of this tile, indexed by column
-    int64_t n, ld, row0, j0, jstep, ncols;
+    int64_t n, ld, row0, row_end, j0, jstep, ncols;
     double eta, omega, nw, aw;
     bool avg_empty, gzero, do_step;
 };
@@ -407,7 +408,8 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const int w = tid / kWave;
     const int64_t n = a.n, ld = a.ld;
     const int64_t i0 = a.row0 + (int64_t)w * WROWS + (int64_t)lane * VEC;
-    const bool lane_active = i0 < n;
+    const int64_t row_end = a.row_end;
+    const bool lane_active = i0 < row_end;
     const bool avg_empty = a.avg_empty;
     const bool do_step = a.do_step;
     const T* __restrict__ C = a.C;
@@ -426,7 +428,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     bool valid[VEC];
 #pragma unroll
     for (int e = 0; e < VEC; ++e) {
-        valid[e] = (i0 + e) < n;
+        valid[e] = (i0 + e) < row_end;
         ui[e] = (lane_active && valid[e]) ? a.uc[i0 + e] : 0.0;
         uai[e] = (EVAL && lane_active && valid[e]) ? a.ua[i0 + e] : 0.0;
     }

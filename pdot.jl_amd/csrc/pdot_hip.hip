@@ -69,6 +69,7 @@ struct StepParams {
     double* evalpart;
     int64_t n, m, ld;
     int32_t TC, Cb, R;
+    int32_t TRB;         // rows per row block: n split evenly over the R blocks (<= the block's rows)
     int32_t interleave;  // 1: block cb takes columns cb, cb+Cb, ...; 0: a contiguous range
 };
 
@@ -90,7 +91,7 @@ struct ReduceParams {
 // ------------------------------------------------------------- step kernel
 
 // One PDHG pass over the whole problem: block (rb, cb) owns rows
-// [rb*TR, (rb+1)*TR) of the columns cb, cb+Cb, ... (interleaved) or of the
+// [rb*TRB, min(n, (rb+1)*TRB)) of the columns cb, cb+Cb, ... (interleaved) or of the
 // contiguous range [cb*TC, (cb+1)*TC); step_tile does the work.
 template <typename T, bool EVAL, int COST>
 __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave)) void step_kernel(StepParams p) {
@@ -112,7 +113,8 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     TileArgs<T> a;
     a.n = p.n;
     a.ld = p.ld;
-    a.row0 = (int64_t)rb * TR;
+    a.row0 = (int64_t)rb * p.TRB;
+    a.row_end = (a.row0 + p.TRB < p.n) ? a.row0 + p.TRB : p.n;
     a.j0 = ilv ? cb : (int64_t)cb * p.TC;
     a.jstep = ilv ? p.Cb : 1;
     if (ilv) a.ncols = (p.m - cb + p.Cb - 1) / p.Cb;
@@ -401,7 +403,7 @@ struct pdot_ctx {
     double cdiv = 1.0;
     double* px = nullptr;                   // dim x n source coordinates
     double* py = nullptr;                   // m x dim target coordinates
-    int32_t R = 0, Cb = 0, TC = 0, nstep = 0, NBr = 0, NBc = 0, nred = 0, sqchunk = 0;
+    int32_t R = 0, Cb = 0, TC = 0, TRB = 0, nstep = 0, NBr = 0, NBc = 0, nred = 0, sqchunk = 0;
     int32_t cb_hint = 0;
     int32_t interleave = 1;
     bool uploaded = false, started = false;
@@ -484,6 +486,17 @@ int setup_geometry(pdot_ctx* ctx) {
     Cb = (ctx->m + TC - 1) / TC;
     if (R * Cb > 0x7fffffffLL) return fail(ctx, PDOT_E_ARG, "grid too large");
     ctx->R = (int32_t)R;
+#ifndef PDOT_ROW_SPLIT_ALIGN
+#define PDOT_ROW_SPLIT_ALIGN 128  // bytes (measured best of 64, 128, 256 and no split); 0: full-height row blocks
+#endif
+    if (PDOT_ROW_SPLIT_ALIGN > 0) {
+        // even row split with aligned row offsets, so that no row block is short
+        const int64_t align = PDOT_ROW_SPLIT_ALIGN / (int64_t)ctx->esz;
+        const int64_t per = (ctx->n + R - 1) / R;
+        ctx->TRB = (int32_t)std::min<int64_t>(TR, (per + align - 1) / align * align);
+    } else {
+        ctx->TRB = TR;
+    }
     ctx->Cb = (int32_t)Cb;
     ctx->TC = (int32_t)TC;
     ctx->nstep = (int32_t)(R * Cb);
@@ -518,6 +531,7 @@ StepParams step_params(pdot_ctx* ctx) {
     p.m = ctx->m;
     p.ld = ctx->ld;
     p.TC = ctx->TC;
+    p.TRB = ctx->TRB;
     p.Cb = ctx->Cb;
     p.R = ctx->R;
     p.interleave = ctx->interleave;
