@@ -1,0 +1,64 @@
+"""CPU tests of host-side logic that needs no GPU: the warm-start feature and
+target construction, the closed-form first-evaluation KKT error of the zero
+start used by the warm-start judge (pinned against the oracle's first
+evaluation), the point-cloud problem's dense matrix, and the batch tile-width
+rule's inputs."""
+import numpy as np
+import pytest
+
+import oracle as O
+import pdot_amd as P
+from pdot_amd import instances, learned
+
+
+def test_normalize_duals_keeps_the_dual_objective_and_balances():
+    rng = np.random.default_rng(1)
+    a, b = rng.dirichlet(np.ones(30)), rng.dirichlet(np.ones(20))
+    u, v = rng.standard_normal(30), rng.standard_normal(20)
+    un, vn = learned.normalize_duals(u, v, a, b)
+    assert a @ un + b @ vn == pytest.approx(a @ u + b @ v, rel=1e-14)
+    assert a @ un == pytest.approx(b @ vn, abs=1e-14)
+    # invariant under the shift it removes
+    us, vs = learned.normalize_duals(u + 3.0, v - 3.0, a, b)
+    np.testing.assert_allclose(us, un, atol=1e-13)
+    np.testing.assert_allclose(vs, vn, atol=1e-13)
+
+
+def test_features_layout():
+    x, y = np.array([[0.1, 0.2], [0.3, 0.4]]), np.array([[0.5, 0.6], [0.7, 0.8], [0.9, 1.0]])
+    a, b = np.array([0.25, 0.75]), np.array([0.2, 0.3, 0.5])
+    f = learned.features(x, y, a, b)
+    assert f.shape == (5, 4) and f.dtype == np.float32
+    np.testing.assert_allclose(f[:2, :2], x)
+    np.testing.assert_allclose(f[2:, :2], y)
+    np.testing.assert_allclose(f[:2, 2], 2 * a)
+    np.testing.assert_allclose(f[2:, 2], 3 * b)
+    assert list(f[:, 3]) == [1, 1, -1, -1, -1]
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_zero_start_kkt_closed_form_matches_oracle(seed):
+    prob = instances.uniform_sqeuclid(40, 33, seed=seed)
+    r = O.optimize(P.default_parameters(1e-4, iteration_limit=1), prob, mode=O.FUSED)
+    s = r["stats"][0]
+    kkt = max(s["relative_l2_primal_residual"], s["relative_l2_dual_residual"], s["relative_optimality_gap"])
+    assert learned.zero_start_relative_kkt(prob) == pytest.approx(kkt, rel=1e-14)
+
+
+def test_point_cloud_dense_matches_generators():
+    for make_pts, make_dense, args in ((instances.gaussian_shift_points, instances.gaussian_shift, (60, 45, 3)),
+                                       (instances.l1_mixture_3d_points, instances.l1_mixture_3d, (50, 40, 2)),
+                                       (instances.uniform_sqeuclid_points, instances.uniform_sqeuclid, (30, 20, 5))):
+        d = make_pts(*args).dense()
+        e = make_dense(*args)
+        assert np.array_equal(d.cost_matrix, e.cost_matrix)
+        assert np.array_equal(d.source_distribution, e.source_distribution)
+
+
+def test_point_cloud_problem_validation():
+    with pytest.raises(ValueError):
+        P.PointCloudProblem(np.zeros((3, 4)), np.zeros((2, 4)), np.ones(3) / 3, np.ones(2) / 2)
+    with pytest.raises(ValueError):
+        P.PointCloudProblem(np.zeros((3, 2)), np.zeros((2, 2)), np.ones(3) / 3, np.ones(2) / 2, cost="cosine")
+    with pytest.raises(ValueError):
+        P.PointCloudProblem(np.zeros((3, 2)), np.zeros((2, 3)), np.ones(3) / 3, np.ones(2) / 2)
