@@ -360,7 +360,7 @@ __device__ __forceinline__ double point_cost(const double* xi, const double* yj,
             c += (COST == kCostSqEuclid) ? t * t : fabs(t);
         }
     }
-    return c / cdiv;
+    return cdiv == 1.0 ? c : c / cdiv;  // x / 1 == x: skip the division for unscaled costs
 }
 
 // Everything one block needs to advance one tile of one problem by a PDHG

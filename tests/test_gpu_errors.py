@@ -1,0 +1,58 @@
+"""GPU tests of the C ABI's error behaviour: calls out of order or with bad
+arguments return a PDOT_E_* code with a message (raised as PdotError by the
+binding, as the reference raises a Julia error), and leave the context usable."""
+import numpy as np
+import pytest
+
+import pdot_amd as P
+from pdot_amd import _lib, instances
+
+pytestmark = pytest.mark.gpu
+
+
+def test_state_errors_are_reported_and_recoverable():
+    prob = instances.uniform_sqeuclid(50, 40, seed=3)
+    dev = _lib.DeviceSolver(50, 40)
+    with pytest.raises(_lib.PdotError, match="before pdot_upload"):
+        dev.cached_ot_info()
+    with pytest.raises(_lib.PdotError, match="before pdot_upload"):
+        dev.set_start()
+    dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+    for call in (lambda: dev.run_steps(3), lambda: dev.eval(), lambda: dev.download(),
+                 lambda: dev.round(), lambda: dev.time_steps(2)):
+        with pytest.raises(_lib.PdotError, match="before pdot_set_start"):
+            call()
+    dev.set_start()
+    with pytest.raises(_lib.PdotError, match="average is empty"):
+        dev.download(_lib.POINT_AVERAGE)
+    with pytest.raises(_lib.PdotError, match="average is empty"):
+        dev.round(_lib.POINT_AVERAGE)
+    # still usable: a solve on the same context matches a fresh one
+    out = P.optimize(P.default_parameters(1e-4), prob, solver=dev)
+    ref = P.optimize(P.default_parameters(1e-4), prob)
+    assert out.iteration_count == ref.iteration_count
+    assert np.array_equal(out.primal_solution, ref.primal_solution)
+    dev.close()
+
+
+def test_argument_errors():
+    dev = _lib.DeviceSolver(20, 30)
+    with pytest.raises(ValueError):
+        dev.upload(np.zeros((30, 20)), np.ones(20) / 20, np.ones(30) / 30)
+    bad = P.PointCloudProblem(np.zeros((20, 2)), np.zeros((30, 2)), np.ones(20) / 20, np.ones(30) / 30)
+    with pytest.raises(_lib.PdotError, match="cannot normalise a zero cost"):
+        dev.upload_points(P.PointCloudProblem(bad.source_points, bad.target_points, bad.source_distribution,
+                                              bad.target_distribution, normalize=True))
+    with pytest.raises(_lib.PdotError, match="unknown point"):
+        dev.upload_points(bad)
+        dev.set_start()
+        dev.download(7)
+    dev.close()
+    with pytest.raises(_lib.PdotError):
+        _lib.DeviceSolver(0, 10)
+
+
+def test_round_matrix_rejects_bad_shapes():
+    prob = instances.uniform_sqeuclid(10, 8, seed=1)
+    with pytest.raises(ValueError):
+        P.rounding(prob, np.zeros((8, 10)))
