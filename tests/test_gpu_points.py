@@ -25,7 +25,10 @@ POINTS = {"uniform_sqeuclid": instances.uniform_sqeuclid_points, "gaussian_shift
 
 @pytest.mark.parametrize("make,args", [(instances.gaussian_shift_points, (700, 500, 3)),
                                        (instances.l1_mixture_3d_points, (400, 450, 4)),
-                                       (instances.uniform_sqeuclid_points, (300, 257, 11))])
+                                       (instances.uniform_sqeuclid_points, (300, 257, 11)),
+                                       (instances.uniform_sqeuclid_points, (1, 37, 2)),
+                                       (instances.gaussian_shift_points, (41, 1, 6)),
+                                       (instances.l1_mixture_3d_points, (1031, 17, 8))])
 def test_point_solve_equals_dense_solve_bitwise(make, args):
     pts = make(*args)
     dense = pts.dense()
