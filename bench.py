@@ -200,7 +200,7 @@ def bench_batched(args) -> int:
 def bench_warm_start(args) -> int:
     """Held-out 4096 x 4096 Gaussian-mixture instances (FP32 storage): time to
     1e-4 relative KKT from the zero start and through the learned warm start
-    (committed network weights, KKT judge and zero-start fallback), the
+    (committed network weights, KKT acceptance test and zero-start fallback), the
     prediction time included."""
     import numpy as np
     import pdot_amd as P
@@ -254,7 +254,7 @@ def main() -> int:
                          "FP32-storage stress case and the 512-instance batch")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)
         return 2
     if args.workload == "l1_mixture_3d":

@@ -2,9 +2,9 @@
 predicts the dual potentials (u, v) of a transport problem from its point
 clouds and masses; the prediction only sets the starting iterate of the
 same PDHG solve (X0 = 0, G0 = (C - u0) - v0).  Zero-start PDHG stays the
-fallback and the solver's own termination test is the judge: a warm-started
-solve that does not reach OPTIMAL within the budget of the zero start is
-rerun from zero.
+fallback and the solver's own KKT error decides: a predicted start whose
+first evaluation is not better than the zero start's, or whose solve does
+not reach OPTIMAL, is replaced by the zero-start solve.
 
 The network runs on the GPU through pdot_mlp (libpdot_hip.so): bf16 MFMA
 matrix products with FP32 accumulation, trained with Adam by the library's
@@ -208,7 +208,7 @@ def zero_start_relative_kkt(problem) -> float:
 
 def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True):
     """optimize() from the network's duals, with zero-start PDHG as the
-    fallback and the solver's own KKT check as the judge:
+    fallback and the solver's own KKT check deciding:
 
     1. the predicted start is evaluated (the first evaluation of a solve
        limited to one iteration); if its relative KKT error is not below the

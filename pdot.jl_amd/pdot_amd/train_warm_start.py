@@ -53,7 +53,7 @@ def generate_training_data(count, n_min, n_max, tol, seed, batch_size=512, itera
 
 def evaluate(net, count, n, tols=(1e-4,), storage=_lib.STORAGE_F64, log=print):
     """Held-out comparison: iterations and time to each tolerance from the
-    zero start and through learned.solve_warm (prediction, KKT judge,
+    zero start and through learned.solve_warm (prediction, KKT acceptance test,
     fallback), same solver and parameters.  Warm solves are capped at 20x the
     zero start's iterations (then reported as not reaching the tolerance)."""
     from dataclasses import replace

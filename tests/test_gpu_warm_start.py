@@ -6,7 +6,7 @@ training and evaluation seeds of the training script).
 * Prediction: the predicted dual potentials are closer to the converged
   duals than the zero start is (mean squared error after the (u + t, v - t)
   normalisation), by at least 2x.
-* Solve: the warm-started solve, with the KKT judge and the zero-start
+* Solve: the warm-started solve, with the KKT acceptance test and the zero-start
   fallback of learned.solve_warm, terminates OPTIMAL at the same objective
   as the zero start.  The iteration counts of both are printed: the
   restarted adaptive PDHG does not turn the better dual start into fewer
@@ -52,7 +52,7 @@ def test_network_predicts_duals_better_than_zero(net):
     assert max(ratios) < 0.5
 
 
-def test_warm_started_solve_is_judged_and_correct(net):
+def test_warm_started_solve_is_checked_and_correct(net):
     tol = 1e-4
     for seed in HELD_OUT:
         inst = instances.gmm_dirichlet(512, seed=seed)

@@ -1,6 +1,6 @@
 """CPU tests of host-side logic that needs no GPU: the warm-start feature and
 target construction, the closed-form first-evaluation KKT error of the zero
-start used by the warm-start judge (pinned against the oracle's first
+start used by the warm start's acceptance test (pinned against the oracle's first
 evaluation), the point-cloud problem's dense matrix, and the batch tile-width
 rule's inputs."""
 import numpy as np
