@@ -300,16 +300,22 @@ template <> struct VecT<float> {
 #ifndef PDOT_NT
 #define PDOT_NT 1
 #endif
-// 16-byte loads/stores of the streamed matrices; with PDOT_NT the accesses
-// carry the nontemporal hint (the matrices are touched once per pass).
+#ifndef PDOT_NT_LOAD
+#define PDOT_NT_LOAD PDOT_NT
+#endif
+#ifndef PDOT_NT_STORE
+#define PDOT_NT_STORE PDOT_NT
+#endif
+// 16-byte loads/stores of the streamed matrices; with the nontemporal hint
+// (the matrices are touched once per pass).
 template <typename V>
 __device__ __forceinline__ V ld16(const V* p) {
-    if constexpr (PDOT_NT) return __builtin_nontemporal_load(p);
+    if constexpr (PDOT_NT_LOAD) return __builtin_nontemporal_load(p);
     else return *p;
 }
 template <typename V>
 __device__ __forceinline__ void st16(V v, V* p) {
-    if constexpr (PDOT_NT) __builtin_nontemporal_store(v, p);
+    if constexpr (PDOT_NT_STORE) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
 
