@@ -90,7 +90,7 @@ def solve_to_tol(prob, storage, tol, dev=None, do_round=True):
     out = P.optimize(P.default_parameters(tol, verbosity=0, record_iteration_stats=False), prob,
                      storage=storage, solver=dev, profile=prof)
     ci = out.iteration_stats[-1].convergence_information[0]
-    res = {"seconds": prof.solve_sec, "iterations": out.iteration_count,
+    res = {"seconds": prof.solve_sec, "output_copy_sec": prof.output_sec, "iterations": out.iteration_count,
            "termination": out.termination_string, "objective": ci.primal_objective,
            "rel_kkt": P.relative_kkt_error(ci), "evaluations": prof.evaluations, "restarts": prof.restarts}
     if dev is not None and do_round:

@@ -27,6 +27,7 @@ from .termination import cached_ot_info
 class BatchProfile:
     upload_sec: float = 0.0
     solve_sec: float = 0.0
+    output_sec: float = 0.0
     evaluations: int = 0
     passes_launched: int = 0
     finish_sec: List[float] = field(default_factory=list)
@@ -109,13 +110,7 @@ def optimize_batch(params: PdhgParameters, problems: Sequence[OptimalTransportPr
                 actions[k] = 2 if decision.to_average else 1
                 weights[k] = w
         if done_now.any():
-            batch.discard(done_now)
-            for k in np.flatnonzero(done_now):
-                reason, it_k, avg_count = outputs[k]
-                which = _lib.POINT_CURRENT if avg_count == 0 else _lib.POINT_AVERAGE
-                X, u, v = batch.download(int(k), which)
-                outputs[k] = SaddlePointOutput(X, u, v, reason, termination_reason_to_string(reason), it_k - 1,
-                                               ctls[k].iteration_stats)
+            batch.discard(done_now)  # finished instances keep their output point resident
             active[done_now == 1] = 0
             batch.set_active(active)
             if not active.any():
@@ -130,9 +125,17 @@ def optimize_batch(params: PdhgParameters, problems: Sequence[OptimalTransportPr
         time_basic += time.time() - t0
         passes += int(counts.max()) if counts.size else 0
         iteration = nxt
+    solve_sec = time.time() - start_time
+    for k in range(count):  # outputs copied once every instance has terminated
+        reason, it_k, avg_count = outputs[k]
+        which = _lib.POINT_CURRENT if avg_count == 0 else _lib.POINT_AVERAGE
+        X, u, v = batch.download(int(k), which)
+        outputs[k] = SaddlePointOutput(X, u, v, reason, termination_reason_to_string(reason), it_k - 1,
+                                       ctls[k].iteration_stats)
     if profile is not None:
         profile.upload_sec = upload_sec
-        profile.solve_sec = time.time() - start_time
+        profile.solve_sec = solve_sec
+        profile.output_sec = time.time() - start_time - solve_sec
         profile.evaluations = max(c.evaluations for c in ctls)
         profile.passes_launched = passes
         profile.finish_sec = finish

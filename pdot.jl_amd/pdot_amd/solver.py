@@ -166,10 +166,13 @@ def display_iteration_stats(stats: IterationStats) -> None:
 @dataclass
 class SolveProfile:
     """Timing side-channel of one optimize call (not part of the reference
-    output): wall time from after the upload to the terminating evaluation,
-    and the number of evaluations."""
+    output): wall time from after the upload to the terminating evaluation
+    (where the reference reads cumulative_time_sec, solver_gpu.jl:501-620),
+    the device-to-host copy of the output after it, and the number of
+    evaluations."""
     upload_sec: float = 0.0
     solve_sec: float = 0.0
+    output_sec: float = 0.0
     evaluations: int = 0
     restarts: int = 0
     take_step_calls: int = 0
@@ -312,13 +315,15 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
         if print_to_screen_this_iteration(reason, iteration, params.verbosity, freq):
             display_iteration_stats(stats)
         if reason is not None:
+            solve_sec = time.time() - start_time
             solver.discard()
             which = _lib.POINT_CURRENT if st.avg_count == 0 else _lib.POINT_AVERAGE
             solver.output_point = which  # the returned point stays resident (rounding on the device)
             X, u, v = solver.download(which)
             if profile is not None:
                 profile.upload_sec = upload_sec
-                profile.solve_sec = time.time() - start_time
+                profile.solve_sec = solve_sec
+                profile.output_sec = time.time() - start_time - solve_sec
                 profile.evaluations = ctl.evaluations
                 profile.restarts = ctl.restarts
                 profile.take_step_calls = int(st.take_step_calls)
