@@ -305,3 +305,18 @@ def test_fp32_storage_objective_within_1e_6_of_tight_fp64():
     o64 = tight.iteration_stats[-1].convergence_information[0].primal_objective
     o32 = f32.iteration_stats[-1].convergence_information[0].primal_objective
     assert abs(o32 - o64) / abs(o64) < 1e-6, (o32, o64)
+
+
+@pytest.mark.parametrize("kind", ["constant_step", "fixed_frequency", "no_restarts", "no_restart_to_current",
+                                  "scale_invariant", "linf"])
+def test_policy_variants_full_solve_results(kind):
+    """Each policy variant solved to 1e-4: OPTIMAL, and the objective within
+    10 tol (1 + |p*|) of the HiGHS optimum of the fixture instance."""
+    case = next(c for c in FIX["lp"] if c["generator"] == "uniform_sqeuclid" and (c["n"], c["m"]) == (300, 257))
+    prob = instances.uniform_sqeuclid(case["n"], case["m"], seed=case["seed"])
+    params = _variant_params(kind)
+    params.termination_criteria.iteration_limit = 300000
+    out = P.optimize(params, prob)
+    assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL, (kind, out.iteration_count)
+    pobj = float(np.sum(prob.cost_matrix * out.primal_solution))
+    assert abs(pobj - case["optimum"]) <= 10 * 1e-4 * (1 + abs(case["optimum"]))
