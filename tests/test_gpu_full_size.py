@@ -90,3 +90,24 @@ def test_l1_mixture_30k_fp32_certified_gap():
     lb, ub = _certificate(cols, a, b, out.dual_source_solution, lambda j0, j1: Y[:, j0:j1])
     cinf = max(float(cols(j0, min(30000, j0 + 1000)).max()) for j0 in range(0, 30000, 1000))
     _check_certificate(out, lb, ub, a, b, cinf, _lib.STORAGE_F32)
+
+
+def test_batched_uniform_instances_certified():
+    """The first 48 instances of the 512-instance batch workload (n, m in
+    [64, 2048], FP32 storage) solved together: every OPTIMAL instance carries
+    the same certificate (rounding of the downloaded plan on the device)."""
+    probs = instances.batched_uniform(512)[:48]
+    from pdot_amd.batch import optimize_batch
+    outs = optimize_batch(P.default_parameters(1e-4, iteration_limit=100000, record_iteration_stats=False), probs,
+                          storage=_lib.STORAGE_F32)
+    optimal = 0
+    for prob, out in zip(probs, outs):
+        if out.termination_reason != P.TerminationReason.TERMINATION_REASON_OPTIMAL:
+            assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_ITERATION_LIMIT
+            continue
+        optimal += 1
+        C, a, b = prob.cost_matrix, prob.source_distribution, prob.target_distribution
+        Y = P.rounding(prob, out.primal_solution.copy(order="F"))
+        lb, ub = _certificate(lambda j0, j1: C[:, j0:j1], a, b, out.dual_source_solution, lambda j0, j1: Y[:, j0:j1])
+        _check_certificate(out, lb, ub, a, b, float(np.abs(C).max()), _lib.STORAGE_F32)
+    assert optimal >= 45
