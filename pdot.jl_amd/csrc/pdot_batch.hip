@@ -408,6 +408,7 @@ struct pdot_batch {
     int32_t step_grid = 0, reduce_grid = 0, grid_mult = 1;
     int32_t step_resident = 0, reduce_resident = 0;
     bool started = false;
+    bool layout_ok = false;  // partial arenas and work lists match the active set
     bool use_graph = false;
     std::map<int64_t, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> events;
@@ -556,6 +557,7 @@ int rebuild_work(pdot_batch* b) {
 }
 
 int setup_batch_partials(pdot_batch* b) {
+    b->layout_ok = false;  // stays false if an allocation below fails
     const int64_t nd = layout_partials(b);
     int rc;
     if (nd > b->part_cap || !b->part) {
@@ -591,6 +593,7 @@ int setup_batch_partials(pdot_batch* b) {
     if (sg != b->step_grid || rg != b->reduce_grid) bfree_graphs(b);  // captured launches hold the grid
     b->step_grid = sg;
     b->reduce_grid = rg;
+    b->layout_ok = true;
     return PDOT_OK;
 }
 
@@ -856,6 +859,7 @@ int pdot_batch_set_states(pdot_batch* b, const pdot_solver_state* in) {
 int pdot_batch_run_steps(pdot_batch* b, const int64_t* counts, pdot_solver_state* out) {
     if (!b || !counts) return PDOT_E_ARG;
     if (!b->started) return bfail(b, PDOT_E_STATE, "pdot_batch_run_steps before pdot_batch_set_start");
+    if (!b->layout_ok) return bfail(b, PDOT_E_STATE, "pdot_batch_run_steps: the last re-layout failed: " + b->err);
     BHIPCHK(b, hipSetDevice(b->device));
     int rc = bread_states(b);
     if (rc) return rc;
@@ -895,6 +899,7 @@ int pdot_batch_run_steps(pdot_batch* b, const int64_t* counts, pdot_solver_state
 int pdot_batch_eval(pdot_batch* b, pdot_point_stats* cur, pdot_point_stats* avg, pdot_solver_state* states) {
     if (!b) return PDOT_E_ARG;
     if (!b->started) return bfail(b, PDOT_E_STATE, "pdot_batch_eval before pdot_batch_set_start");
+    if (!b->layout_ok) return bfail(b, PDOT_E_STATE, "pdot_batch_eval: the last re-layout failed: " + b->err);
     BHIPCHK(b, hipSetDevice(b->device));
     BHIPCHK(b, blaunch_commit(b));
     BHIPCHK(b, blaunch_step(b, true));
@@ -996,6 +1001,7 @@ int pdot_batch_download(pdot_batch* b, int32_t k, int32_t which, double* X, doub
 int pdot_batch_time_steps(pdot_batch* b, int64_t count, double* step_kernel_ms, double* total_ms) {
     if (!b) return PDOT_E_ARG;
     if (!b->started) return bfail(b, PDOT_E_STATE, "pdot_batch_time_steps before pdot_batch_set_start");
+    if (!b->layout_ok) return bfail(b, PDOT_E_STATE, "pdot_batch_time_steps: the last re-layout failed: " + b->err);
     if (count < 1) return bfail(b, PDOT_E_ARG, "pdot_batch_time_steps: count < 1");
     BHIPCHK(b, hipSetDevice(b->device));
     int rc = bread_states(b);

@@ -469,7 +469,7 @@ void free_partials(pdot_ctx* ctx) {
 // Launch geometry: R row blocks of rows_per_block rows; Cb column blocks.
 int setup_geometry(pdot_ctx* ctx) {
     free_graphs(ctx);
-    free_partials(ctx);
+    free_partials(ctx);  // evalvec, allocated last, stays null if an allocation below fails
     const int TR = rows_per_block(ctx->storage, ctx->cost_kind);
     const int64_t R = (ctx->n + TR - 1) / TR;
     int64_t Cb;
@@ -993,6 +993,7 @@ int pdot_set_start(pdot_ctx* ctx, const double* u0, const double* v0, int32_t st
 int pdot_run_steps(pdot_ctx* ctx, int64_t count, pdot_solver_state* out) {
     if (!ctx) return PDOT_E_ARG;
     if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_run_steps before pdot_set_start");
+    if (!ctx->evalvec) return fail(ctx, PDOT_E_STATE, "pdot_run_steps: launch buffers missing after a failed allocation");
     if (count < 0) return fail(ctx, PDOT_E_ARG, "pdot_run_steps: negative count");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     if (ctx->use_graph && count > 1) {
@@ -1021,6 +1022,7 @@ int pdot_run_steps(pdot_ctx* ctx, int64_t count, pdot_solver_state* out) {
 int pdot_eval(pdot_ctx* ctx, pdot_point_stats* current, pdot_point_stats* average, pdot_solver_state* state) {
     if (!ctx) return PDOT_E_ARG;
     if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_eval before pdot_set_start");
+    if (!ctx->evalvec) return fail(ctx, PDOT_E_STATE, "pdot_eval: launch buffers missing after a failed allocation");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, launch_commit(ctx));
     HIPCHK(ctx, launch_step(ctx, true));
@@ -1175,6 +1177,7 @@ int pdot_round(pdot_ctx* ctx, int32_t which, double* X_rounded) {
 int pdot_time_steps(pdot_ctx* ctx, int64_t count, double* step_kernel_ms, double* total_ms) {
     if (!ctx) return PDOT_E_ARG;
     if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_time_steps before pdot_set_start");
+    if (!ctx->evalvec) return fail(ctx, PDOT_E_STATE, "pdot_time_steps: launch buffers missing after a failed allocation");
     if (count < 1) return fail(ctx, PDOT_E_ARG, "pdot_time_steps: count < 1");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t need = (size_t)(2 * count + 2);
