@@ -19,8 +19,10 @@
 // own step size, primal weight, averaging weights, restart and termination
 // flags, and a step budget: a pass is performed only while the budget is
 // positive, so instances advance by different numbers of passes in one
-// launch sequence.  The last reduce chunk of an instance to finish (per-
-// instance atomic completion counter) runs that instance's controller.
+// launch sequence.  After the reduce kernel, bfinish_kernel runs each active
+// instance's controller over its chunks' scalars (one wave per instance; the
+// kernel boundary publishes them, so no chunk runs a device-scope fence, an
+// L2 write-back and invalidate on this GPU).
 
 #include <hip/hip_runtime.h>
 
@@ -41,11 +43,11 @@ constexpr int kBWaves = 4;                 // batched step kernel: waves per blo
 #endif
 constexpr int kBPipe = PDOT_BATCH_PIPE;  // column batches prefetched ahead in a tile
 constexpr int kBBlock = kWave * kBWaves;
-constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block
+constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block (8 measured no faster)
 constexpr int kBRedBlock = kWave * kBRedWaves;
 constexpr int kMinColTiles = 32;           // column tiles per instance (at least)
 constexpr int kHeavyParts = 32;            // chunks with more partials are reduced by a whole block
-constexpr int kMinTileCols = 4;            // columns per tile (at least)
+constexpr int kMinTileCols = 4;            // columns per tile (at least; 8 measured slower)
 
 struct BInst {
     int64_t n, m, ld;
@@ -73,10 +75,10 @@ struct BParams {
     double* red;   // per-instance pass reductions and eval vector partials (fixed layout)
     const BInst* inst;
     pdot_solver_state* st;
-    unsigned int* counter;
     const BTile* tiles;
     const BChunk* chunks;
-    const int32_t* nwork;  // {ntiles, heavy chunks, light chunks}
+    const int32_t* nwork;  // {ntiles, heavy chunks, light chunks, active instances}
+    const int32_t* actlist;  // the active instances
     double* evalout;       // kEvalOut per instance
 };
 
@@ -157,14 +159,13 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
     }
 }
 
-// Persistent reduce kernel over the chunk list; the last chunk of an instance
-// to finish commits (or, for an evaluation, marks pending) that instance.
+// Persistent reduce kernel over the chunk list: every chunk leaves its three
+// scalars (and, with EVAL, its evaluation vector parts) for bfinish_kernel.
 template <bool EVAL>
 __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
     __shared__ pdot_solver_state sst;
     __shared__ BInst sinst;
     __shared__ BChunk sch;
-    __shared__ int amlast;
     __shared__ double psum[kBRedWaves][kWave];
     const int tid = threadIdx.x;
     const int nheavy = p.nwork[1], nlight = p.nwork[2];
@@ -174,7 +175,6 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
             sch = p.chunks[t];
             sinst = p.inst[sch.inst];
             sst = p.st[sch.inst];
-            amlast = 0;
         }
         __syncthreads();
         const bool do_step = !sst.numerical_error && (EVAL || sst.step_budget > 0);
@@ -220,28 +220,12 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
                     rp[0] = out3[0];
                     rp[1] = out3[1];
                     rp[2] = out3[2];
-                    __threadfence();
-                    const unsigned prev = atomicAdd(p.counter + sch.inst, 1u);
-                    amlast = (prev == (unsigned)sinst.nred - 1) ? 1 : 0;
-                }
-            }
-            __syncthreads();
-            if (amlast) {
-                __threadfence();
-                if (tid < kWave) {
-                    if (!EVAL) reduce_and_commit(&sst, p.red + sinst.redpart, sinst.nred);
-                    if (tid == 0) {
-                        if (EVAL) sst.pending = 1;
-                        p.st[sch.inst] = sst;
-                        p.counter[sch.inst] = 0u;
-                    }
                 }
             }
         }
         __syncthreads();
     }
     // the other chunks: one wave each, no barriers
-    __shared__ pdot_solver_state wst[kBRedWaves];
     const int w = tid / kWave, lane = tid & (kWave - 1);
     for (int item = blockIdx.x * kBRedWaves + w; item < nlight; item += gridDim.x * kBRedWaves) {
         const BChunk sch = p.chunks[nheavy + item];
@@ -281,7 +265,6 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
         c.do_step = do_step;
         double out3[3], ev[kVecStats];
         reduce_chunk_wave<EVAL>(c, out3, ev);
-        int last = 0;
         if (lane == 0) {
             if (EVAL) {
                 for (int q = 0; q < kVecStats; ++q) p.red[in.evalvec + lc * kVecStats + q] = ev[q];
@@ -291,23 +274,6 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
                 rp[0] = out3[0];
                 rp[1] = out3[1];
                 rp[2] = out3[2];
-                __threadfence();
-                const unsigned prev = atomicAdd(p.counter + sch.inst, 1u);
-                last = (prev == (unsigned)in.nred - 1) ? 1 : 0;
-            }
-        }
-        last = __shfl(last, 0);
-        if (last) {
-            // every chunk of the instance has finished: commit its pass
-            __threadfence();
-            if (lane == 0) wst[w] = p.st[sch.inst];
-            __builtin_amdgcn_wave_barrier();
-            if (!EVAL) reduce_and_commit(&wst[w], p.red + in.redpart, in.nred);
-            __builtin_amdgcn_wave_barrier();
-            if (lane == 0) {
-                if (EVAL) wst[w].pending = 1;
-                p.st[sch.inst] = wst[w];
-                p.counter[sch.inst] = 0u;
             }
         }
     }
@@ -321,6 +287,26 @@ __global__ __launch_bounds__(kWave) void bcommit_kernel(BParams p, int32_t count
     if (threadIdx.x == 0) sst = p.st[k];
     __syncthreads();
     if (!sst.pending) return;
+    const BInst& in = p.inst[k];
+    reduce_and_commit(&sst, p.red + in.redpart, in.nred);
+    if (threadIdx.x == 0) p.st[k] = sst;
+}
+
+// Commits the pass of every active instance that stepped, from its chunks'
+// scalars; an evaluation's pass is left pending for the host's decision
+// instead (one wave per instance).
+template <bool EVAL>
+__global__ __launch_bounds__(kWave) void bfinish_kernel(BParams p) {
+    __shared__ pdot_solver_state sst;
+    if ((int)blockIdx.x >= p.nwork[3]) return;
+    const int k = p.actlist[blockIdx.x];
+    if (threadIdx.x == 0) sst = p.st[k];
+    __syncthreads();
+    if (sst.numerical_error || (!EVAL && sst.step_budget <= 0)) return;
+    if (EVAL) {
+        if (threadIdx.x == 0) p.st[k].pending = 1;
+        return;
+    }
     const BInst& in = p.inst[k];
     reduce_and_commit(&sst, p.red + in.redpart, in.nred);
     if (threadIdx.x == 0) p.st[k] = sst;
@@ -391,11 +377,11 @@ struct pdot_batch {
     double* red = nullptr;
     BInst* d_inst = nullptr;
     pdot_solver_state* d_st = nullptr;
-    unsigned int* d_counter = nullptr;
     BTile* d_tiles = nullptr;
     BChunk* d_chunks = nullptr;
     int32_t* d_nwork = nullptr;
     int32_t* d_active = nullptr;
+    int32_t* d_actlist = nullptr;
     double* d_evalout = nullptr;
     double* stage = nullptr;
     int64_t stage_elems = 0;
@@ -403,9 +389,9 @@ struct pdot_batch {
     double* h_eval = nullptr;           // pinned, count * kEvalOut
     std::vector<int32_t> active;
     int64_t max_tiles = 0, max_chunks = 0, tile_cap = 0;
-    int32_t ntiles = 0, nchunks = 0, nheavy = 0;
+    int32_t ntiles = 0, nchunks = 0, nheavy = 0, nact = 0;
     int32_t cols_per_tile = 64;
-    int32_t step_grid = 0, reduce_grid = 0, grid_mult = 1;
+    int32_t step_grid = 0, reduce_grid = 0, finish_grid = 0, grid_mult = 1;
     int32_t step_resident = 0, reduce_resident = 0;
     bool started = false;
     bool layout_ok = false;  // partial arenas and work lists match the active set
@@ -450,10 +436,10 @@ BParams bparams(pdot_batch* b) {
     p.red = b->red;
     p.inst = b->d_inst;
     p.st = b->d_st;
-    p.counter = b->d_counter;
     p.tiles = b->d_tiles;
     p.chunks = b->d_chunks;
     p.nwork = b->d_nwork;
+    p.actlist = b->d_actlist;
     p.evalout = b->d_evalout;
     return p;
 }
@@ -521,8 +507,10 @@ int rebuild_work(pdot_batch* b) {
     std::vector<BChunk> chunks, heavy, light;
     const int TR = rows_per_tile(b->storage);
     std::vector<std::pair<int64_t, BTile>> weighted;
+    std::vector<int32_t> actlist;
     for (int32_t k = 0; k < b->count; ++k) {
         if (!b->active[k]) continue;
+        actlist.push_back(k);
         const BInst& in = b->inst[k];
         for (int32_t cb = 0; cb < in.Cb; ++cb) {
             const int64_t cols = std::min<int64_t>(in.TC, in.m - (int64_t)cb * in.TC);  // tile work for the sort
@@ -545,13 +533,16 @@ int rebuild_work(pdot_batch* b) {
     b->ntiles = (int32_t)tiles.size();
     b->nchunks = (int32_t)chunks.size();
     b->nheavy = (int32_t)heavy.size();
-    int32_t nw[3] = {b->ntiles, b->nheavy, (int32_t)light.size()};
+    b->nact = (int32_t)actlist.size();
+    int32_t nw[4] = {b->ntiles, b->nheavy, (int32_t)light.size(), b->nact};
     BHIPCHK(b, hipStreamSynchronize(b->stream));
     if (!tiles.empty())
         BHIPCHK(b, hipMemcpy(b->d_tiles, tiles.data(), sizeof(BTile) * tiles.size(), hipMemcpyHostToDevice));
     if (!chunks.empty())
         BHIPCHK(b, hipMemcpy(b->d_chunks, chunks.data(), sizeof(BChunk) * chunks.size(), hipMemcpyHostToDevice));
     BHIPCHK(b, hipMemcpy(b->d_nwork, nw, sizeof(nw), hipMemcpyHostToDevice));
+    if (!actlist.empty())
+        BHIPCHK(b, hipMemcpy(b->d_actlist, actlist.data(), sizeof(int32_t) * actlist.size(), hipMemcpyHostToDevice));
     BHIPCHK(b, hipMemcpy(b->d_active, b->active.data(), sizeof(int32_t) * b->count, hipMemcpyHostToDevice));
     return PDOT_OK;
 }
@@ -590,9 +581,11 @@ int setup_batch_partials(pdot_batch* b) {
     const int32_t sg = (int32_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)b->step_resident * b->grid_mult, b->ntiles));
     const int32_t reduce_items = b->nheavy + (b->nchunks - b->nheavy + kBRedWaves - 1) / kBRedWaves;
     const int32_t rg = std::max(1, std::min(b->reduce_resident, reduce_items));
-    if (sg != b->step_grid || rg != b->reduce_grid) bfree_graphs(b);  // captured launches hold the grid
+    const int32_t fg = std::max(1, b->nact);
+    if (sg != b->step_grid || rg != b->reduce_grid || fg != b->finish_grid) bfree_graphs(b);  // captured launches hold the grid
     b->step_grid = sg;
     b->reduce_grid = rg;
+    b->finish_grid = fg;
     b->layout_ok = true;
     return PDOT_OK;
 }
@@ -613,6 +606,10 @@ hipError_t blaunch_reduce(pdot_batch* b, bool eval) {
     BParams p = bparams(b);
     if (eval) hipLaunchKernelGGL((breduce_kernel<true>), dim3(b->reduce_grid), dim3(kBRedBlock), 0, b->stream, p);
     else hipLaunchKernelGGL((breduce_kernel<false>), dim3(b->reduce_grid), dim3(kBRedBlock), 0, b->stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (eval) hipLaunchKernelGGL((bfinish_kernel<true>), dim3(b->finish_grid), dim3(kWave), 0, b->stream, p);
+    else hipLaunchKernelGGL((bfinish_kernel<false>), dim3(b->finish_grid), dim3(kWave), 0, b->stream, p);
     return hipGetLastError();
 }
 
@@ -712,16 +709,15 @@ int pdot_batch_create(pdot_batch** out, int32_t count, const int64_t* n, const i
     BHIPCHK(b, hipMemsetAsync(b->vec, 0, sizeof(double) * (size_t)voff, b->stream));
     if ((rc = balloc(b, (void**)&b->d_inst, sizeof(BInst) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_st, sizeof(pdot_solver_state) * count))) return rc;
-    if ((rc = balloc(b, (void**)&b->d_counter, sizeof(unsigned int) * count))) return rc;
-    if ((rc = balloc(b, (void**)&b->d_nwork, sizeof(int32_t) * 3))) return rc;
+    if ((rc = balloc(b, (void**)&b->d_nwork, sizeof(int32_t) * 4))) return rc;
     if ((rc = balloc(b, (void**)&b->d_active, sizeof(int32_t) * count))) return rc;
+    if ((rc = balloc(b, (void**)&b->d_actlist, sizeof(int32_t) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_evalout, sizeof(double) * kEvalOut * count))) return rc;
     const int64_t nred_total = layout_reductions(b);
     if ((rc = balloc(b, (void**)&b->red, sizeof(double) * (size_t)nred_total))) return rc;
     b->max_chunks = 0;
     for (const auto& in : b->inst) b->max_chunks += in.nred;
     if ((rc = balloc(b, (void**)&b->d_chunks, sizeof(BChunk) * (size_t)b->max_chunks))) return rc;
-    BHIPCHK(b, hipMemsetAsync(b->d_counter, 0, sizeof(unsigned int) * count, b->stream));
     BHIPCHK(b, hipMemsetAsync(b->d_st, 0, sizeof(pdot_solver_state) * count, b->stream));
     BHIPCHK(b, hipHostMalloc((void**)&b->h_st, sizeof(pdot_solver_state) * count, hipHostMallocDefault));
     BHIPCHK(b, hipHostMalloc((void**)&b->h_eval, sizeof(double) * kEvalOut * count, hipHostMallocDefault));
@@ -745,8 +741,8 @@ void pdot_batch_destroy(pdot_batch* b) {
     if (b->stream) (void)hipStreamSynchronize(b->stream);
     bfree_graphs(b);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
-    void* ps[] = {b->mat, b->vec, b->part, b->red, b->d_inst, b->d_st, b->d_counter, b->d_tiles, b->d_chunks,
-                  b->d_nwork, b->d_active, b->d_evalout, b->stage};
+    void* ps[] = {b->mat, b->vec, b->part, b->red, b->d_inst, b->d_st, b->d_tiles, b->d_chunks,
+                  b->d_nwork, b->d_active, b->d_actlist, b->d_evalout, b->stage};
     for (void* q : ps)
         if (q) (void)hipFree(q);
     if (b->h_st) (void)hipHostFree(b->h_st);
@@ -807,7 +803,6 @@ int pdot_batch_set_start(pdot_batch* b, int32_t step_policy, double reduction_ex
         st.spare0 = 1;
         st.spare1 = 2;
     }
-    BHIPCHK(b, hipMemsetAsync(b->d_counter, 0, sizeof(unsigned int) * b->count, b->stream));
     int rc = bwrite_states(b);
     if (rc) return rc;
     b->active.assign(b->count, 1);

@@ -706,10 +706,13 @@ __device__ __forceinline__ void finish_chunk(const ChunkArgs& c, int lane, doubl
     }
 }
 
-// Sum of partials t0 <= t < t1 at position k, with eight independent
-// accumulators (eight loads in flight per lane) combined in a fixed order.
+// Sum of partials t0 <= t < t1 at position k, with A independent
+// accumulators (A loads in flight per lane) combined by a fixed pairwise
+// tree; the last, partial group is loaded at once and added to the first
+// accumulator in order.
+template <int A = 8>
 __device__ __forceinline__ double sum_partials(const ChunkArgs& c, int64_t k, int t0, int t1) {
-    constexpr int A = 8;
+    static_assert(A >= 2 && (A & (A - 1)) == 0, "accumulators: a power of two");
     double acc[A];
 #pragma unroll
     for (int q = 0; q < A; ++q) acc[q] = 0.0;
@@ -718,8 +721,20 @@ __device__ __forceinline__ double sum_partials(const ChunkArgs& c, int64_t k, in
 #pragma unroll
         for (int q = 0; q < A; ++q) acc[q] += c.part[(int64_t)(t + q) * c.pstride + k];
     }
-    for (; t < t1; ++t) acc[0] += c.part[(int64_t)t * c.pstride + k];
-    return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    if (t < t1) {
+        double v[A];
+#pragma unroll
+        for (int q = 0; q < A; ++q) v[q] = (t + q < t1) ? c.part[(int64_t)(t + q) * c.pstride + k] : 0.0;
+#pragma unroll
+        for (int q = 0; q < A; ++q)
+            if (t + q < t1) acc[0] += v[q];
+    }
+#pragma unroll
+    for (int st = 1; st < A; st *= 2) {
+#pragma unroll
+        for (int q = 0; q + st < A; q += 2 * st) acc[q] += acc[q + st];
+    }
+    return acc[0];
 }
 
 // Finishes one pass for a chunk (src/solver_gpu.jl:172-173, 177-234,
@@ -729,7 +744,7 @@ __device__ __forceinline__ double sum_partials(const ChunkArgs& c, int64_t k, in
 // split into RWAVES contiguous ranges (one per wave), then combined in a fixed
 // order by wave 0, which finishes the chunk.  Called by all threads of the
 // block; thread 0 receives out3 = {dot, yy, dx} and ev_out[kVecStats].
-template <bool EVAL, int RWAVES>
+template <bool EVAL, int RWAVES, int A = 8>
 __device__ __forceinline__ void reduce_chunk(const ChunkArgs& c, double (*psum)[kWave], double* out3, double* ev_out) {
     const int tid = threadIdx.x;
     const int lane = tid & (kWave - 1);
@@ -740,7 +755,7 @@ __device__ __forceinline__ void reduce_chunk(const ChunkArgs& c, double (*psum)[
         if (k < c.len) {
             const int t0 = (int)(((int64_t)c.np * w) / RWAVES);
             const int t1 = (int)(((int64_t)c.np * (w + 1)) / RWAVES);
-            s = sum_partials(c, k, t0, t1);
+            s = sum_partials<A>(c, k, t0, t1);
         }
         psum[w][lane] = s;
     }

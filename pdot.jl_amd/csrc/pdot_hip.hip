@@ -20,10 +20,13 @@
 //   reduce_kernel  finishes the row/column sums (fixed order), applies the
 //                  dual update, the averaging of u, v, r, c and the vector
 //                  parts of interaction and movement.
-//   the controller (the adaptive step-size recurrence, the replay of a
-//                  rejected step and the commit) runs in the last block of
-//                  reduce_kernel to finish (an atomic completion counter), or
-//                  in commit_kernel for a pass left pending by an evaluation.
+//   finish_kernel  one wave: the controller (the adaptive step-size
+//                  recurrence, the replay of a rejected step and the commit)
+//                  over the reduce blocks' scalars; commit_kernel does the
+//                  same for a pass left pending by an evaluation.  The kernel
+//                  boundary publishes the reduce blocks' results: no
+//                  device-scope fence (an L2 write-back and invalidate on
+//                  this GPU) runs per block.
 // Evaluation of the point and its average (src/utils_gpu.jl:68-129) is fused
 // into the step kernel as extra reductions of the values it already loads.
 //
@@ -78,7 +81,6 @@ struct ReduceParams {
     const double* a;
     const double* b;
     pdot_solver_state* st;
-    unsigned int* counter;
     const double* rowpart;
     const double* colpart;
     const double* sqpart;
@@ -161,19 +163,14 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
 // ----------------------------------------------------------- reduce kernel
 
 // Block b < NBr owns rows [64b, 64b+64), the next NBc blocks the columns;
-// reduce_chunk does the work.  The last block to finish (atomic completion
-// counter) runs the controller, which commits the pass; an evaluation's pass
-// is left pending for the host's decision instead.
+// reduce_chunk does the work; each block leaves its three scalars in
+// redpart for finish_kernel.
 template <bool EVAL>
 __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     __shared__ pdot_solver_state sst;
-    __shared__ int amlast;
     __shared__ double psum[kRedWaves][kWave];
     const int tid = threadIdx.x;
-    if (tid == 0) {
-        sst = *p.st;
-        amlast = 0;
-    }
+    if (tid == 0) sst = *p.st;
     __syncthreads();
     const bool do_step = !sst.numerical_error;
     if (!do_step && !EVAL) return;
@@ -223,24 +220,6 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
             p.redpart[3 * blockIdx.x + 0] = out3[0];
             p.redpart[3 * blockIdx.x + 1] = out3[1];
             p.redpart[3 * blockIdx.x + 2] = out3[2];
-            // publish the partials, then count this block as done
-            __threadfence();
-            const unsigned prev = atomicAdd(p.counter, 1u);
-            amlast = (prev == gridDim.x - 1) ? 1 : 0;
-        }
-    }
-    if (!do_step) return;  // halted: evaluation only, nothing to commit
-    __syncthreads();
-    if (!amlast) return;
-    // Last block: every other block has published its partials and read the
-    // state, so the state can be updated in place.
-    __threadfence();
-    if (tid < kWave) {
-        if (!EVAL) reduce_and_commit(&sst, p.redpart, p.nred);
-        if (tid == 0) {
-            if (EVAL) sst.pending = 1;  // an evaluation's pass waits for the host's decision
-            *p.st = sst;
-            *p.counter = 0u;
         }
     }
 }
@@ -270,6 +249,23 @@ __global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart,
         acc = is_max ? block_max(acc, red) : block_sum(acc, red);
         if (tid == 0) out[kMatStats + k] = acc;
     }
+}
+
+// Commits the pass the reduce kernel has reduced; an evaluation's pass is
+// left pending for the host's decision instead (one wave).  A halted solve
+// (numerical error) commits nothing.
+template <bool EVAL>
+__global__ __launch_bounds__(kWave) void finish_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
+    __shared__ pdot_solver_state sst;
+    if (threadIdx.x == 0) sst = *st;
+    __syncthreads();
+    if (sst.numerical_error) return;
+    if (EVAL) {
+        if (threadIdx.x == 0) st->pending = 1;
+        return;
+    }
+    reduce_and_commit(&sst, redpart, nred);
+    if (threadIdx.x == 0) *st = sst;
 }
 
 // Commits a pending pass (one wave).
@@ -387,7 +383,6 @@ struct pdot_ctx {
     double* stage = nullptr;
     int64_t stage_cols = 0;
     pdot_solver_state* stA = nullptr;
-    unsigned int* counter = nullptr;      // reduce-kernel completion counter
     int32_t resident_blocks = 0;          // step-kernel blocks resident at once
     pdot_solver_state* h_state = nullptr;  // pinned
     double* h_eval = nullptr;              // pinned
@@ -544,7 +539,6 @@ ReduceParams reduce_params(pdot_ctx* ctx) {
     p.a = ctx->a;
     p.b = ctx->b;
     p.st = ctx->stA;
-    p.counter = ctx->counter;
     p.rowpart = ctx->rowpart;
     p.colpart = ctx->colpart;
     p.sqpart = ctx->sqpart;
@@ -590,6 +584,10 @@ hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
     dim3 grid(ctx->nred), block(kRedBlock);
     if (eval) hipLaunchKernelGGL((reduce_kernel<true>), grid, block, 0, ctx->stream, p);
     else hipLaunchKernelGGL((reduce_kernel<false>), grid, block, 0, ctx->stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (eval) hipLaunchKernelGGL((finish_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
+    else hipLaunchKernelGGL((finish_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
     return hipGetLastError();
 }
 
@@ -762,8 +760,6 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     if ((rc = dev_alloc(ctx, (void**)&ctx->b, sizeof(double) * (size_t)m))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalout, sizeof(double) * kEvalOut))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->stA, sizeof(pdot_solver_state)))) return rc;
-    if ((rc = dev_alloc(ctx, (void**)&ctx->counter, sizeof(unsigned int)))) return rc;
-    HIPCHK(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(unsigned int), ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_state, sizeof(pdot_solver_state), hipHostMallocDefault));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_eval, sizeof(double) * kEvalOut, hipHostMallocDefault));
     if ((rc = configure_kernel(ctx))) return rc;
@@ -777,7 +773,7 @@ void pdot_destroy(pdot_ctx* ctx) {
     free_graphs(ctx);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     free_partials(ctx);
-    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->counter, ctx->stage};
+    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->stage};
     for (void* q : ps)
         if (q) hipFree(q);
     for (int s = 0; s < 4; ++s) {
