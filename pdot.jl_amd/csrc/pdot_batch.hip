@@ -37,11 +37,19 @@
 
 namespace {
 
-constexpr int kBWaves = 4;                 // batched step kernel: waves per block
+// Batched step kernel: waves per block, each block one tile of 64 x VEC rows
+// (256 in FP32).  Instances have n in the tens to thousands of rows; short
+// tiles waste fewer lanes on the last row tile of an instance.  Measured at
+// the 512-instance workload: one wave 2.35 ms per all-active pass, two
+// 2.41, four 2.39; with about a dozen instances left 44, 45 and 53 us.
+#ifndef PDOT_BATCH_WAVES
+#define PDOT_BATCH_WAVES 1
+#endif
+constexpr int kBWaves = PDOT_BATCH_WAVES;
 #ifndef PDOT_BATCH_PIPE
 #define PDOT_BATCH_PIPE 0
 #endif
-constexpr int kBPipe = PDOT_BATCH_PIPE;  // column batches prefetched ahead in a tile
+constexpr int kBPipe = PDOT_BATCH_PIPE;  // column batches prefetched ahead in a tile (1 measured slower)
 constexpr int kBBlock = kWave * kBWaves;
 constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block (8 measured no faster)
 constexpr int kBRedBlock = kWave * kBRedWaves;
@@ -225,9 +233,12 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
         }
         __syncthreads();
     }
-    // the other chunks: one wave each, no barriers
+    // the other chunks: one wave each, no barriers.  The block order is
+    // rotated so that the light chunks start at the first block without a
+    // heavy chunk: when the grid covers all the work, no block does both.
     const int w = tid / kWave, lane = tid & (kWave - 1);
-    for (int item = blockIdx.x * kBRedWaves + w; item < nlight; item += gridDim.x * kBRedWaves) {
+    const int vb = (int)((blockIdx.x + gridDim.x - (unsigned)nheavy % gridDim.x) % gridDim.x);
+    for (int item = vb * kBRedWaves + w; item < nlight; item += gridDim.x * kBRedWaves) {
         const BChunk sch = p.chunks[nheavy + item];
         const BInst& in = p.inst[sch.inst];
         const pdot_solver_state& gs = p.st[sch.inst];
