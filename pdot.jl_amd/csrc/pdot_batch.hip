@@ -309,8 +309,11 @@ __global__ __launch_bounds__(kWave) void bcommit_kernel(BParams p, int32_t count
 template <bool EVAL>
 __global__ __launch_bounds__(kWave) void bfinish_kernel(BParams p) {
     __shared__ pdot_solver_state sst;
-    if ((int)blockIdx.x >= p.nwork[3]) return;
+    // the grid never exceeds the instance count, so the list entry is in
+    // bounds and is loaded together with the list length
+    const int nact = p.nwork[3];
     const int k = p.actlist[blockIdx.x];
+    if ((int)blockIdx.x >= nact) return;
     if (threadIdx.x == 0) sst = p.st[k];
     __syncthreads();
     if (sst.numerical_error || (!EVAL && sst.step_budget <= 0)) return;
