@@ -50,6 +50,10 @@ constexpr int kBWaves = PDOT_BATCH_WAVES;
 #define PDOT_BATCH_PIPE 0
 #endif
 constexpr int kBPipe = PDOT_BATCH_PIPE;  // column batches prefetched ahead in a tile (1 measured slower)
+#ifndef PDOT_BATCH_UNROLL
+#define PDOT_BATCH_UNROLL 2
+#endif
+constexpr int kBUnroll = PDOT_BATCH_UNROLL;  // columns per load batch (1 slower, 4 no faster)
 constexpr int kBBlock = kWave * kBWaves;
 constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block (8 measured no faster)
 constexpr int kBRedBlock = kWave * kBRedWaves;
@@ -154,7 +158,7 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             a.gzero = sst.gzero != 0;
             a.do_step = do_step;
             double dd = 0.0, ev[kMatStats];
-            step_tile<T, EVAL, 2, kBWaves, kBPipe>(a, colbuf, red, &dd, ev);
+            step_tile<T, EVAL, kBUnroll, kBWaves, kBPipe>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
                 const int64_t lb = stl.rb + (int64_t)stl.cb * sinst.R;
                 if (do_step) p.part[sinst.sqpart + lb] = dd;
