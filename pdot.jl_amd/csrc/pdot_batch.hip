@@ -59,6 +59,7 @@ constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per b
 constexpr int kBRedBlock = kWave * kBRedWaves;
 constexpr int kMinColTiles = 32;           // column tiles per instance (at least)
 constexpr int kHeavyParts = 32;            // chunks with more partials are reduced by a whole block
+
 constexpr int kMinTileCols = 4;            // columns per tile (at least; 8 measured slower)
 
 struct BInst {
@@ -171,6 +172,43 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
     }
 }
 
+// The arguments of chunk `chunk` (64 rows, or 64 columns) of instance `in`
+// in state `s`; *lc receives the chunk's index within the instance (rows
+// first, then columns).
+__device__ __forceinline__ ChunkArgs chunk_args(const BParams& p, const BInst& in, const pdot_solver_state& s,
+                                                bool rows, int32_t chunk, bool do_step, int64_t* lc) {
+    const bool avg_empty = s.avg_count == 0;
+    const int cs = s.cur_slot, as = avg_empty ? cs : s.avg_slot;
+    const int64_t* yo = rows ? in.u : in.v;
+    const int64_t* so = rows ? in.r : in.c;
+    ChunkArgs c;
+    c.rows = rows;
+    c.k0 = (int64_t)chunk * kWave;
+    c.len = rows ? in.n : in.m;
+    c.part = p.part + (rows ? in.rowpart : in.colpart);
+    c.pstride = rows ? in.ld : in.m;
+    c.np = rows ? in.Cb : in.R;
+    c.q = p.vec + (rows ? in.a : in.b);
+    c.y_cur = p.vec + yo[cs];
+    c.s_cur = p.vec + so[cs];
+    c.y_avg = p.vec + yo[as];
+    c.s_avg = p.vec + so[as];
+    c.y_new = p.vec + yo[s.spare0];
+    c.s_new = p.vec + so[s.spare0];
+    c.y_anew = p.vec + yo[s.spare1];
+    c.s_anew = p.vec + so[s.spare1];
+    *lc = rows ? chunk : in.NBr + chunk;
+    c.sqpart = p.part + in.sqpart;
+    c.sq_lo = *lc * in.sqchunk;
+    c.sq_hi = c.sq_lo + in.sqchunk < in.nstep ? c.sq_lo + in.sqchunk : in.nstep;
+    c.eta = s.step_size;
+    c.omega = s.primal_weight;
+    avg_weights(s, &c.nw, &c.aw);
+    c.avg_empty = avg_empty;
+    c.do_step = do_step;
+    return c;
+}
+
 // Persistent reduce kernel over the chunk list: every chunk leaves its three
 // scalars (and, with EVAL, its evaluation vector parts) for bfinish_kernel.
 template <bool EVAL>
@@ -191,36 +229,8 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
         __syncthreads();
         const bool do_step = !sst.numerical_error && (EVAL || sst.step_budget > 0);
         if (do_step || EVAL) {
-            const bool rows = sch.rows != 0;
-            const bool avg_empty = sst.avg_count == 0;
-            const int cs = sst.cur_slot, as = avg_empty ? cs : sst.avg_slot;
-            const int64_t* yo = rows ? sinst.u : sinst.v;
-            const int64_t* so = rows ? sinst.r : sinst.c;
-            ChunkArgs c;
-            c.rows = rows;
-            c.k0 = (int64_t)sch.chunk * kWave;
-            c.len = rows ? sinst.n : sinst.m;
-            c.part = p.part + (rows ? sinst.rowpart : sinst.colpart);
-            c.pstride = rows ? sinst.ld : sinst.m;
-            c.np = rows ? sinst.Cb : sinst.R;
-            c.q = p.vec + (rows ? sinst.a : sinst.b);
-            c.y_cur = p.vec + yo[cs];
-            c.s_cur = p.vec + so[cs];
-            c.y_avg = p.vec + yo[as];
-            c.s_avg = p.vec + so[as];
-            c.y_new = p.vec + yo[sst.spare0];
-            c.s_new = p.vec + so[sst.spare0];
-            c.y_anew = p.vec + yo[sst.spare1];
-            c.s_anew = p.vec + so[sst.spare1];
-            const int64_t lc = rows ? sch.chunk : sinst.NBr + sch.chunk;  // chunk index within the instance
-            c.sqpart = p.part + sinst.sqpart;
-            c.sq_lo = lc * sinst.sqchunk;
-            c.sq_hi = c.sq_lo + sinst.sqchunk < sinst.nstep ? c.sq_lo + sinst.sqchunk : sinst.nstep;
-            c.eta = sst.step_size;
-            c.omega = sst.primal_weight;
-            avg_weights(sst, &c.nw, &c.aw);
-            c.avg_empty = avg_empty;
-            c.do_step = do_step;
+            int64_t lc;
+            const ChunkArgs c = chunk_args(p, sinst, sst, sch.rows != 0, sch.chunk, do_step, &lc);
             double out3[3], ev[kVecStats];
             reduce_chunk<EVAL, kBRedWaves>(c, psum, out3, ev);
             if (tid == 0) {
@@ -243,41 +253,13 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
     const int w = tid / kWave, lane = tid & (kWave - 1);
     const int vb = (int)((blockIdx.x + gridDim.x - (unsigned)nheavy % gridDim.x) % gridDim.x);
     for (int item = vb * kBRedWaves + w; item < nlight; item += gridDim.x * kBRedWaves) {
-        const BChunk sch = p.chunks[nheavy + item];
-        const BInst& in = p.inst[sch.inst];
-        const pdot_solver_state& gs = p.st[sch.inst];
+        const BChunk ch = p.chunks[nheavy + item];
+        const BInst& in = p.inst[ch.inst];
+        const pdot_solver_state& gs = p.st[ch.inst];
         const bool do_step = !gs.numerical_error && (EVAL || gs.step_budget > 0);
         if (!do_step && !EVAL) continue;
-        const bool rows = sch.rows != 0;
-        const bool avg_empty = gs.avg_count == 0;
-        const int cs = gs.cur_slot, as = avg_empty ? cs : gs.avg_slot;
-        const int64_t* yo = rows ? in.u : in.v;
-        const int64_t* so = rows ? in.r : in.c;
-        ChunkArgs c;
-        c.rows = rows;
-        c.k0 = (int64_t)sch.chunk * kWave;
-        c.len = rows ? in.n : in.m;
-        c.part = p.part + (rows ? in.rowpart : in.colpart);
-        c.pstride = rows ? in.ld : in.m;
-        c.np = rows ? in.Cb : in.R;
-        c.q = p.vec + (rows ? in.a : in.b);
-        c.y_cur = p.vec + yo[cs];
-        c.s_cur = p.vec + so[cs];
-        c.y_avg = p.vec + yo[as];
-        c.s_avg = p.vec + so[as];
-        c.y_new = p.vec + yo[gs.spare0];
-        c.s_new = p.vec + so[gs.spare0];
-        c.y_anew = p.vec + yo[gs.spare1];
-        c.s_anew = p.vec + so[gs.spare1];
-        const int64_t lc = rows ? sch.chunk : in.NBr + sch.chunk;  // chunk index within the instance
-        c.sqpart = p.part + in.sqpart;
-        c.sq_lo = lc * in.sqchunk;
-        c.sq_hi = c.sq_lo + in.sqchunk < in.nstep ? c.sq_lo + in.sqchunk : in.nstep;
-        c.eta = gs.step_size;
-        c.omega = gs.primal_weight;
-        avg_weights(gs, &c.nw, &c.aw);
-        c.avg_empty = avg_empty;
-        c.do_step = do_step;
+        int64_t lc;
+        const ChunkArgs c = chunk_args(p, in, gs, ch.rows != 0, ch.chunk, do_step, &lc);
         double out3[3], ev[kVecStats];
         reduce_chunk_wave<EVAL>(c, out3, ev);
         if (lane == 0) {
