@@ -19,7 +19,7 @@
 // own step size, primal weight, averaging weights, restart and termination
 // flags, and a step budget: a pass is performed only while the budget is
 // positive, so instances advance by different numbers of passes in one
-// launch sequence.  After the reduce kernel, bfinish_kernel runs each active
+// launch sequence.  After the reduce kernel, bcontroller_kernel runs each active
 // instance's controller over its chunks' scalars (one wave per instance; the
 // kernel boundary publishes them, so no chunk runs a device-scope fence, an
 // L2 write-back and invalidate on this GPU).
@@ -210,7 +210,7 @@ __device__ __forceinline__ ChunkArgs chunk_args(const BParams& p, const BInst& i
 }
 
 // Persistent reduce kernel over the chunk list: every chunk leaves its three
-// scalars (and, with EVAL, its evaluation vector parts) for bfinish_kernel.
+// scalars (and, with EVAL, its evaluation vector parts) for bcontroller_kernel.
 template <bool EVAL>
 __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
     __shared__ pdot_solver_state sst;
@@ -293,7 +293,7 @@ __global__ __launch_bounds__(kWave) void bcommit_kernel(BParams p, int32_t count
 // scalars; an evaluation's pass is left pending for the host's decision
 // instead (one wave per instance).
 template <bool EVAL>
-__global__ __launch_bounds__(kWave) void bfinish_kernel(BParams p) {
+__global__ __launch_bounds__(kWave) void bcontroller_kernel(BParams p) {
     __shared__ pdot_solver_state sst;
     // the grid never exceeds the instance count, so the list entry is in
     // bounds and is loaded together with the list length
@@ -391,7 +391,7 @@ struct pdot_batch {
     int64_t max_tiles = 0, max_chunks = 0, tile_cap = 0;
     int32_t ntiles = 0, nchunks = 0, nheavy = 0, nact = 0;
     int32_t cols_per_tile = 64;
-    int32_t step_grid = 0, reduce_grid = 0, finish_grid = 0, grid_mult = 1;
+    int32_t step_grid = 0, reduce_grid = 0, controller_grid = 0, grid_mult = 1;
     int32_t step_resident = 0, reduce_resident = 0;
     bool started = false;
     bool layout_ok = false;  // partial arenas and work lists match the active set
@@ -582,10 +582,10 @@ int setup_batch_partials(pdot_batch* b) {
     const int32_t reduce_items = b->nheavy + (b->nchunks - b->nheavy + kBRedWaves - 1) / kBRedWaves;
     const int32_t rg = std::max(1, std::min(b->reduce_resident, reduce_items));
     const int32_t fg = std::max(1, b->nact);
-    if (sg != b->step_grid || rg != b->reduce_grid || fg != b->finish_grid) bfree_graphs(b);  // captured launches hold the grid
+    if (sg != b->step_grid || rg != b->reduce_grid || fg != b->controller_grid) bfree_graphs(b);  // captured launches hold the grid
     b->step_grid = sg;
     b->reduce_grid = rg;
-    b->finish_grid = fg;
+    b->controller_grid = fg;
     b->layout_ok = true;
     return PDOT_OK;
 }
@@ -608,8 +608,8 @@ hipError_t blaunch_reduce(pdot_batch* b, bool eval) {
     else hipLaunchKernelGGL((breduce_kernel<false>), dim3(b->reduce_grid), dim3(kBRedBlock), 0, b->stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (eval) hipLaunchKernelGGL((bfinish_kernel<true>), dim3(b->finish_grid), dim3(kWave), 0, b->stream, p);
-    else hipLaunchKernelGGL((bfinish_kernel<false>), dim3(b->finish_grid), dim3(kWave), 0, b->stream, p);
+    if (eval) hipLaunchKernelGGL((bcontroller_kernel<true>), dim3(b->controller_grid), dim3(kWave), 0, b->stream, p);
+    else hipLaunchKernelGGL((bcontroller_kernel<false>), dim3(b->controller_grid), dim3(kWave), 0, b->stream, p);
     return hipGetLastError();
 }
 

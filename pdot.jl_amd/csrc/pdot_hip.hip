@@ -20,7 +20,7 @@
 //   reduce_kernel  finishes the row/column sums (fixed order), applies the
 //                  dual update, the averaging of u, v, r, c and the vector
 //                  parts of interaction and movement.
-//   finish_kernel  one wave: the controller (the adaptive step-size
+//   controller_kernel  one wave: the controller (the adaptive step-size
 //                  recurrence, the replay of a rejected step and the commit)
 //                  over the reduce blocks' scalars; commit_kernel does the
 //                  same for a pass left pending by an evaluation.  The kernel
@@ -164,7 +164,7 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
 
 // Block b < NBr owns rows [64b, 64b+64), the next NBc blocks the columns;
 // reduce_chunk does the work; each block leaves its three scalars in
-// redpart for finish_kernel.
+// redpart for controller_kernel.
 template <bool EVAL>
 __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     __shared__ pdot_solver_state sst;
@@ -255,7 +255,7 @@ __global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart,
 // left pending for the host's decision instead (one wave).  A halted solve
 // (numerical error) commits nothing.
 template <bool EVAL>
-__global__ __launch_bounds__(kWave) void finish_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
+__global__ __launch_bounds__(kWave) void controller_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
     __shared__ pdot_solver_state sst;
     if (threadIdx.x == 0) sst = *st;
     __syncthreads();
@@ -586,8 +586,8 @@ hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
     else hipLaunchKernelGGL((reduce_kernel<false>), grid, block, 0, ctx->stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (eval) hipLaunchKernelGGL((finish_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
-    else hipLaunchKernelGGL((finish_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
+    if (eval) hipLaunchKernelGGL((controller_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
+    else hipLaunchKernelGGL((controller_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
     return hipGetLastError();
 }
 
