@@ -145,6 +145,10 @@ def bench_l1_mixture(args) -> int:
         res["solve"] = solve_to_tol(pts, _lib.STORAGE_F32, 1e-4, dev=dev, do_round=False)
     out["f32_point_cloud"] = res
     dev.close()
+    out["roofline"] = {"bound": "hbm", "achieved": out["f32"]["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": out["f32"]["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": None,
+                       "kernel": "step_kernel<float> (FP32 storage)",
+                       "bytes_per_launch": algorithmic_bytes_per_step(n, m, 4)}
     if not args.no_solve:
         out["value"] = out["f32"]["solve"]["seconds"]
         out["objective_rel_err_f32_vs_f64"] = (abs(out["f32"]["solve"]["objective"] - out["f64"]["solve"]["objective"])
@@ -182,6 +186,10 @@ def bench_batched(args) -> int:
            "generate_sec": gen, "upload_sec": up, "autotune": tune["best"],
            "pass_ms_all_active": tt / steps, "step_kernel_GBs": alg / (ks / steps * 1e-3) / 1e9,
            "bytes_per_pass": alg}
+    res["roofline"] = {"bound": "hbm", "achieved": res["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                       "frac": res["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": None,
+                       "kernel": "bstep_kernel<float> (every instance active)", "bytes_per_launch": alg,
+                       "avg_launch_us": ks / steps * 1e3}
     if not args.no_solve:
         prof = BatchProfile()
         outs = optimize_batch(P.default_parameters(1e-4, record_iteration_stats=False), probs, batch=b,
