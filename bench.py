@@ -205,6 +205,34 @@ def bench_batched(args) -> int:
     return 0
 
 
+def bench_precision(args) -> int:
+    """FP32 storage against the FP64 oracle at 10000 x 10000 (gaussian shift):
+    the FP64 solve tightened to 1e-8 and the FP32-storage solve to 1.5e-8,
+    each with the progress table printed every 50 evaluations.  About 15
+    minutes on one MI355X."""
+    import numpy as np
+    import pdot_amd as P
+    from pdot_amd import _lib, instances
+    n = args.n
+    prob = instances.gaussian_shift(n)
+    out = {"metric": "objective of FP32 storage solved to 1.5e-8 vs the FP64 solve to 1e-8 (relative)",
+           "unit": "relative", "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
+           "config": {"workload": f"gaussian_shift {n}x{n}", "n": n, "m": n}}
+    for name, storage, tol in (("f64_oracle", _lib.STORAGE_F64, 1e-8), ("f32", _lib.STORAGE_F32, 1.5e-8)):
+        params = P.default_parameters(tol, verbosity=3, record_iteration_stats=True)
+        t0 = time.time()
+        sol = P.optimize(params, prob, storage=storage)
+        wall = time.time() - t0
+        ci = sol.iteration_stats[-1].convergence_information[0]
+        out[name] = {"tolerance": tol, "termination": sol.termination_string, "iterations": sol.iteration_count,
+                     "seconds": wall, "rel_kkt": P.relative_kkt_error(ci),
+                     "objective": float(np.sum(prob.cost_matrix * sol.primal_solution))}
+        sys.stdout.flush()
+    out["value"] = abs(out["f32"]["objective"] - out["f64_oracle"]["objective"]) / abs(out["f64_oracle"]["objective"])
+    print(json.dumps(out))
+    return 0
+
+
 def bench_warm_start(args) -> int:
     """Held-out 4096 x 4096 Gaussian-mixture instances (FP32 storage): time to
     1e-4 relative KKT from the zero start and through the learned warm start
@@ -257,9 +285,10 @@ def main() -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="gaussian_shift",
-                    choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform", "gmm_warm_start"),
+                    choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform", "gmm_warm_start", "precision"),
                     help="gaussian_shift: the headline 10k x 10k measurement; the others measure the 30k x 30k "
-                         "FP32-storage stress case and the 512-instance batch")
+                         "FP32-storage stress case, the 512-instance batch, the learned warm start and (about "
+                         "15 minutes) FP32 storage against the FP64 1e-8 solve at 10k x 10k")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -269,6 +298,8 @@ def main() -> int:
         return bench_l1_mixture(args)
     if args.workload == "batched_uniform":
         return bench_batched(args)
+    if args.workload == "precision":
+        return bench_precision(args)
     if args.workload == "gmm_warm_start":
         return bench_warm_start(args)
 
