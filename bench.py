@@ -251,11 +251,15 @@ def bench_warm_start(args) -> int:
         zero = P.optimize(prm, p, storage=_lib.STORAGE_F32)
         tz = time.time() - t0
         t0 = time.time()
-        warm, used, pred = learned.solve_warm(prm, p, inst.x, inst.y, net, storage=_lib.STORAGE_F32)
+        warm, info = learned.solve_warm(prm, p, inst.x, inst.y, net, storage=_lib.STORAGE_F32)
         tw = time.time() - t0
         rows.append({"seed": 900200 + k, "zero_iterations": zero.iteration_count, "zero_sec": tz,
-                     "warm_iterations": warm.iteration_count, "warm_sec": tw, "warm_used": bool(used),
-                     "predict_sec": pred, "warm_termination": warm.termination_string})
+                     "warm_iterations": warm.iteration_count, "warm_sec": tw, "duals_used": info.used_duals,
+                     "initial_primal_weight": info.primal_weight, "fell_back": info.fell_back,
+                     "predict_sec": info.predict_sec, "warm_termination": warm.termination_string,
+                     "objective_rel_diff": abs(warm.iteration_stats[-1].convergence_information[0].primal_objective
+                                               - zero.iteration_stats[-1].convergence_information[0].primal_objective)
+                                           / abs(zero.iteration_stats[-1].convergence_information[0].primal_objective)})
         print(f"# instance {k}: zero {zero.iteration_count} it {tz:.2f}s, warm {warm.iteration_count} it {tw:.2f}s",
               file=sys.stderr, flush=True)
     net.close()
@@ -281,7 +285,7 @@ def main() -> int:
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-iters", type=int, default=80, help="iteration limit of the CPU baseline sample (about 15 s)")
-    ap.add_argument("--instances", type=int, default=4, help="held-out instances of the gmm_warm_start workload")
+    ap.add_argument("--instances", type=int, default=8, help="held-out instances of the gmm_warm_start workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="gaussian_shift",

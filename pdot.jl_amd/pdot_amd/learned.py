@@ -1,10 +1,19 @@
-"""Learned warm start: a point-wise network with mean-pooled set context
-predicts the dual potentials (u, v) of a transport problem from its point
-clouds and masses; the prediction only sets the starting iterate of the
-same PDHG solve (X0 = 0, G0 = (C - u0) - v0).  Zero-start PDHG stays the
-fallback and the solver's own KKT error decides: a predicted start whose
-first evaluation is not better than the zero start's, or whose solve does
-not reach OPTIMAL, is replaced by the zero-start solve.
+"""Learned warm start: the starting state of the same PDHG solve, learned
+from solves of the problem family.
+
+* Duals: a point-wise network with mean-pooled set context predicts the
+  dual potentials (u, v) of a transport problem from its point clouds and
+  masses; they set the starting iterate (X0 = 0, G0 = (C - u0) - v0).
+* Primal weight: the solver state's initial primal weight (the reference's
+  `primal_importance`, src/solver_gpu.jl:486-496, 1.0 by default), set to the
+  value that minimised the total iteration count over calibration solves of
+  the family (train_warm_start.calibrate_primal_weight), stored with the
+  network weights.
+
+Zero-start PDHG with the caller's own parameters stays the fallback and the
+solver's own KKT error decides: predicted duals whose first evaluation is not
+better than the zero start's are not used, and a warm solve that does not
+reach OPTIMAL is replaced by the plain zero-start solve.
 
 The network runs on the GPU through pdot_mlp (libpdot_hip.so): bf16 MFMA
 matrix products with FP32 accumulation, trained with Adam by the library's
@@ -197,6 +206,19 @@ def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
         return z["params"].astype(np.float32)
 
 
+def load_meta(path: str = DEFAULT_WEIGHTS) -> dict:
+    """Everything stored beside the network parameters (training summary and
+    the calibrated initial primal weight)."""
+    with np.load(path, allow_pickle=False) as z:
+        return {k: z[k] for k in z.files if k != "params"}
+
+
+def load_initial_primal_weight(path: str = DEFAULT_WEIGHTS) -> Optional[float]:
+    """The calibrated initial primal weight, or None if the weights file has none."""
+    meta = load_meta(path)
+    return float(meta["initial_primal_weight"]) if "initial_primal_weight" in meta else None
+
+
 def zero_start_relative_kkt(problem) -> float:
     """Relative KKT error of the zero start at its first evaluation, in closed
     form: X = 0, u = v = 0 (so G = 0) leave only the primal residual
@@ -206,22 +228,38 @@ def zero_start_relative_kkt(problem) -> float:
     return q / (1.0 + q)
 
 
-def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True):
-    """optimize() from the network's duals, with zero-start PDHG as the
-    fallback and the solver's own KKT check deciding:
+@dataclass
+class WarmStartInfo:
+    """How solve_warm started the returned solve."""
+    used_duals: bool                 # the predicted duals passed the KKT acceptance test
+    primal_weight: Optional[float]   # the initial primal weight of the returned solve (None: the caller's)
+    fell_back: bool                  # the warm solve missed OPTIMAL and the plain solve was returned
+    predict_sec: float
 
-    1. the predicted start is evaluated (the first evaluation of a solve
-       limited to one iteration); if its relative KKT error is not below the
-       zero start's, the solve runs from zero;
-    2. a warm-started solve that does not reach OPTIMAL is rerun from zero.
 
-    Returns (output, used_warm_start, predict_seconds)."""
+def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True,
+               primal_weight: Optional[float] = None, use_calibrated_weight: bool = True):
+    """optimize() from the learned starting state, with the plain zero-start
+    solve as the fallback and the solver's own KKT check deciding:
+
+    1. the predicted duals are evaluated (the first evaluation of a solve
+       limited to one iteration); if their relative KKT error is not below the
+       zero start's, the solve starts from zero duals;
+    2. the solve starts with the calibrated initial primal weight (or
+       `primal_weight`; use_calibrated_weight=False keeps the caller's);
+    3. a warm solve that does not reach OPTIMAL is replaced by the plain
+       zero-start solve with the caller's parameters.
+
+    Returns (output, info: WarmStartInfo)."""
     from dataclasses import replace
     from . import solver as S
     from .solve_log import TerminationReason
     t0 = time.time()
     u0, v0 = net.predict_duals(x, y, problem.source_distribution, problem.target_distribution)
     pred = time.time() - t0
+    if primal_weight is None and use_calibrated_weight:
+        primal_weight = load_initial_primal_weight()
+    warm_params = params if primal_weight is None else replace(params, primal_importance=float(primal_weight))
     n, m = problem.cost_matrix.shape
     dev = _lib.DeviceSolver(n, m, storage=storage)
     try:
@@ -230,11 +268,10 @@ def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE
                         verbosity=0)
         first = S.optimize(probe, problem, storage=storage, warm_start=(u0, v0), solver=dev)
         warm_kkt = S.relative_kkt_error(first.iteration_stats[0].convergence_information[0])
-        if not (warm_kkt < zero_start_relative_kkt(problem)):
-            return S.optimize(params, problem, storage=storage, solver=dev), False, pred
-        out = S.optimize(params, problem, storage=storage, warm_start=(u0, v0), solver=dev)
+        used = bool(warm_kkt < zero_start_relative_kkt(problem))
+        out = S.optimize(warm_params, problem, storage=storage, warm_start=(u0, v0) if used else None, solver=dev)
         if out.termination_reason == TerminationReason.TERMINATION_REASON_OPTIMAL or not fallback:
-            return out, True, pred
-        return S.optimize(params, problem, storage=storage, solver=dev), False, pred
+            return out, WarmStartInfo(used, primal_weight, False, pred)
+        return S.optimize(params, problem, storage=storage, solver=dev), WarmStartInfo(False, None, True, pred)
     finally:
         dev.close()

@@ -8,7 +8,11 @@
    (zero start) to a tight tolerance; their duals, normalised for the
    (u + t, v - t) invariance, are the targets.
 2. Training: Adam on the GPU through pdot_mlp, one instance per step.
-3. Held-out evaluation on instances of the same family at n = m = 4096 with
+3. Calibration of the initial primal weight (calibrate_primal_weight): the
+   weight of a small grid with the fewest total iterations over zero-start
+   solves of separate calibration instances at n = m = 4096; stored beside
+   the network weights (`--calibrate-only` redoes just this step).
+4. Held-out evaluation on instances of the same family at n = m = 4096 with
    seeds disjoint from training: iterations and time to 1e-4 from the zero
    start and from the predicted duals (same solver, same parameters; only
    the starting iterate differs).
@@ -27,7 +31,32 @@ from .solve_log import TerminationReason
 from .solver import SolveProfile, optimize, relative_kkt_error
 
 TRAIN_SEED0 = 100000
+CALIB_SEED0 = 200000
 EVAL_SEED0 = 900000
+PRIMAL_WEIGHT_GRID = (0.05, 0.1, 0.2, 0.3, 0.5, 1.0)
+
+
+def calibrate_primal_weight(count=16, n=4096, grid=PRIMAL_WEIGHT_GRID, tol=1e-4, storage=_lib.STORAGE_F32, log=print):
+    """The initial primal weight (the reference's primal_importance) that
+    minimises the total iteration count to `tol` over `count` calibration
+    instances of the family (seeds CALIB_SEED0 + k, disjoint from training
+    and evaluation), each solved from the zero start with every weight of
+    `grid`.  Returns (best weight, {weight: [iterations per instance]})."""
+    from dataclasses import replace
+    table = {float(w): [] for w in grid}
+    for k in range(count):
+        p = instances.gmm_dirichlet(n, seed=CALIB_SEED0 + k).problem
+        dev = _lib.DeviceSolver(n, n, storage=storage)
+        dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
+        for w in grid:
+            params = replace(default_parameters(tol, record_iteration_stats=False), primal_importance=float(w))
+            out = optimize(params, p, storage=storage, solver=dev)
+            ok = out.termination_reason == TerminationReason.TERMINATION_REASON_OPTIMAL
+            table[float(w)].append(out.iteration_count if ok else 10 ** 9)
+        dev.close()
+        log(f"calibration {k}: " + " ".join(f"{w:g}:{table[float(w)][-1]}" for w in grid))
+    best = min(table, key=lambda w: (sum(table[w]), w))
+    return best, table
 
 
 def generate_training_data(count, n_min, n_max, tol, seed, batch_size=512, iteration_limit=20000, log=print):
@@ -98,6 +127,14 @@ def evaluate(net, count, n, tols=(1e-4,), storage=_lib.STORAGE_F64, log=print):
     return rows, summary
 
 
+def calibration_meta(best, table) -> dict:
+    weights = sorted(table)
+    return {"initial_primal_weight": float(best),
+            "calibration_grid": np.array(weights, dtype=np.float64),
+            "calibration_iterations": np.array([table[w] for w in weights], dtype=np.int64),
+            "calibration_seed0": CALIB_SEED0}
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--train", type=int, default=1024, help="training instances")
@@ -111,8 +148,19 @@ def main(argv=None) -> int:
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=learned.DEFAULT_WEIGHTS)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--calibrate", type=int, default=16, help="calibration instances for the initial primal weight")
+    ap.add_argument("--calibrate-only", action="store_true",
+                    help="keep the network weights in --out and only (re)calibrate the initial primal weight")
     args = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)  # noqa: E731
+
+    if args.calibrate_only:
+        params, meta = learned.load_params(args.out), learned.load_meta(args.out)
+        best, table = calibrate_primal_weight(args.calibrate, log=log)
+        meta.update(calibration_meta(best, table))
+        learned.save_params(args.out, params, meta)
+        log(json.dumps({"initial_primal_weight": best, "total_iterations": {w: sum(v) for w, v in table.items()}}))
+        return 0
 
     t0 = time.time()
     samples = generate_training_data(args.train, args.n_min, args.n_max, args.data_tol, args.seed, log=log)
@@ -130,7 +178,10 @@ def main(argv=None) -> int:
     val_loss = learned.evaluate_loss(net, val)
     log(f"validation mse {val_loss:.4e} (predicting zero: {zero:.4e})")
     params = net.get_params()
-    learned.save_params(args.out, params, {"train_instances": len(trn), "val_mse": val_loss, "epochs": args.epochs})
+    meta = {"train_instances": len(trn), "val_mse": val_loss, "epochs": args.epochs}
+    if args.calibrate > 0:
+        meta.update(calibration_meta(*calibrate_primal_weight(args.calibrate, log=log)))
+    learned.save_params(args.out, params, meta)
     rows, summary = evaluate(net, args.eval, args.eval_n, log=log) if args.eval > 0 else ([], {})
     report = {"data_sec": data_sec, "train_sec": train_sec, "train_instances": len(trn), "val_instances": len(val),
               "train_loss_history": hist, "val_mse": val_loss, "val_mse_zero_prediction": zero,

@@ -62,3 +62,17 @@ def test_point_cloud_problem_validation():
         P.PointCloudProblem(np.zeros((3, 2)), np.zeros((2, 2)), np.ones(3) / 3, np.ones(2) / 2, cost="cosine")
     with pytest.raises(ValueError):
         P.PointCloudProblem(np.zeros((3, 2)), np.zeros((2, 3)), np.ones(3) / 3, np.ones(2) / 2)
+
+
+def test_calibrated_primal_weight_is_stored_with_the_network():
+    """The committed weights carry the calibration of the initial primal
+    weight: the stored weight is the grid value with the fewest total
+    iterations over the calibration solves, and it beats the default 1.0."""
+    meta = learned.load_meta()
+    w = learned.load_initial_primal_weight()
+    grid = [float(g) for g in meta["calibration_grid"]]
+    totals = meta["calibration_iterations"].sum(axis=1)
+    assert w in grid
+    assert totals[grid.index(w)] == totals.min()
+    assert totals[grid.index(w)] < totals[grid.index(1.0)]
+    assert int(meta["calibration_seed0"]) == 200000
