@@ -200,6 +200,24 @@ def bench_batched(args) -> int:
                     "instances_per_s": 512 / prof.solve_sec, "iterations_median": float(np.median(iters)),
                     "iterations_max": int(iters.max()), "passes_launched": prof.passes_launched,
                     "evaluations": prof.evaluations})
+        # the same batch with the initial primal weight calibrated for the learned warm start
+        # (primal_importance; the reference's default, used above, is 1.0)
+        from dataclasses import replace
+        from pdot_amd import learned
+        w0 = learned.load_initial_primal_weight()
+        if w0 is not None:
+            prof2 = BatchProfile()
+            outs2 = optimize_batch(replace(P.default_parameters(1e-4, record_iteration_stats=False),
+                                           primal_importance=w0), probs, batch=b, use_graphs=True, profile=prof2)
+            it2 = np.array([o.iteration_count for o in outs2])
+            obj = np.array([o.iteration_stats[-1].convergence_information[0].primal_objective for o in outs])
+            obj2 = np.array([o.iteration_stats[-1].convergence_information[0].primal_objective for o in outs2])
+            res["with_primal_importance"] = {
+                "primal_importance": w0, "solve_sec": prof2.solve_sec,
+                "optimal": int(sum(o.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+                                   for o in outs2)),
+                "iterations_median": float(np.median(it2)), "iterations_max": int(it2.max()),
+                "objective_rel_diff_max": float(np.max(np.abs(obj2 - obj) / np.abs(obj)))}
     b.close()
     print(json.dumps(res))
     return 0
