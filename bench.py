@@ -84,11 +84,14 @@ def cpu_baseline(prob, iters: int):
                       f"{len(r['stats'])} evaluations)"}
 
 
-def solve_to_tol(prob, storage, tol, dev=None, do_round=True):
+def solve_to_tol(prob, storage, tol, dev=None, do_round=True, primal_importance=None):
+    from dataclasses import replace
     import pdot_amd as P
     prof = P.SolveProfile()
-    out = P.optimize(P.default_parameters(tol, verbosity=0, record_iteration_stats=False), prob,
-                     storage=storage, solver=dev, profile=prof)
+    params = P.default_parameters(tol, verbosity=0, record_iteration_stats=False)
+    if primal_importance is not None:
+        params = replace(params, primal_importance=float(primal_importance))
+    out = P.optimize(params, prob, storage=storage, solver=dev, profile=prof)
     ci = out.iteration_stats[-1].convergence_information[0]
     res = {"seconds": prof.solve_sec, "output_copy_sec": prof.output_sec, "iterations": out.iteration_count,
            "termination": out.termination_string, "objective": ci.primal_objective,
@@ -371,6 +374,17 @@ def main() -> int:
         solves["speedup_f32_vs_f64"] = solves["f64"]["seconds"] / solves["f32"]["seconds"]
         solves["objective_rel_diff_f32_vs_f64"] = (abs(solves["f32"]["objective"] - solves["f64"]["objective"])
                                                    / abs(solves["f64"]["objective"]))
+        # the learned starting state's initial primal weight (calibrated on Gaussian-mixture
+        # solves; the reference's default primal_importance is 1.0)
+        from pdot_amd import learned
+        w0 = learned.load_initial_primal_weight()
+        if w0 is not None:
+            for name, storage in (("f64", _lib.STORAGE_F64), ("f32", _lib.STORAGE_F32)):
+                r = solve_to_tol(prob, storage, 1e-4, dev=devs[name], do_round=False, primal_importance=w0)
+                r["primal_importance"] = w0
+                solves[f"{name}_calibrated_primal_weight"] = r
+            solves["speedup_calibrated_f64_vs_f64"] = solves["f64"]["seconds"] / solves["f64_calibrated_primal_weight"]["seconds"]
+            solves["speedup_calibrated_f32_vs_f64"] = solves["f64"]["seconds"] / solves["f32_calibrated_primal_weight"]["seconds"]
     copy_probe = None
     if args.copy_probe_mib:
         nbytes = args.copy_probe_mib << 20
