@@ -95,3 +95,14 @@ def test_learned_start_halves_iterations_on_held_out_4096(net):
               f"(duals used: {info.used_duals}, primal weight {info.primal_weight})")
     print(f"total: zero {total_zero} it, learned start {total_warm} it ({total_zero / total_warm:.2f}x)")
     assert total_zero >= 2 * total_warm
+
+
+def test_primal_weight_calibration_picks_the_fewest_iterations():
+    """calibrate_primal_weight on two small calibration instances: the table
+    holds one iteration count per instance and weight, and the weight it
+    returns has the smallest total."""
+    from pdot_amd import train_warm_start as T
+    best, table = T.calibrate_primal_weight(count=2, n=384, grid=(0.2, 1.0), log=lambda s: None)
+    assert set(table) == {0.2, 1.0}
+    assert all(len(v) == 2 and all(it > 0 for it in v) for v in table.values())
+    assert sum(table[best]) == min(sum(v) for v in table.values())
