@@ -2,9 +2,10 @@
 // advanced together on one MI355X (the batched form of the PDHG hot path).
 //
 // Layout: every instance has its own C, four point slots (X, u, v, r, c) and
-// partial-sum buffers, carved out of three arenas (matrices in the storage
-// type, vectors and partials in FP64).  Per-instance descriptors (BInst) and
-// solver states (pdot_solver_state) live in device arrays.
+// partial-sum buffers, carved out of four arenas (matrices in the storage
+// type; vectors, per-pass partials and the per-instance pass reductions in
+// FP64).  Per-instance descriptors (BInst) and solver states
+// (pdot_solver_state) live in device arrays.
 //
 // Work lists: a pass over the batch is a list of step tiles (instance, row
 // block, column block) and a list of reduce chunks (instance, 64 rows or
