@@ -84,3 +84,22 @@ def test_point_first_ten_evaluations_match_oracle():
         ci = s_dev.convergence_information[0]
         for f in ("primal_objective", "dual_objective", "l2_primal_residual", "l2_dual_residual"):
             np.testing.assert_allclose(getattr(ci, f), s_ref[f], rtol=1e-9, atol=1e-15, err_msg=f)
+
+
+def test_point_form_beyond_2_31_elements_known_answer():
+    """70000 x 31000 (2.17e9 elements, beyond 32-bit indexing), FP64 storage,
+    point-cloud cost: the third evaluation reports ||X2||_F and <C, X2> of
+    the closed-form plan after pass 2 (tests/point_kat.py), to 1e-11."""
+    from point_kat import third_evaluation
+    n, m = 70000, 31000
+    assert n * m > 2 ** 31
+    g = np.random.default_rng(11)
+    pts = P.PointCloudProblem(g.normal(size=(n, 2)), g.normal(size=(m, 2)) + [2.0, 0.0],
+                              g.random(n) * (2.0 / n), g.random(m) * (2.0 / m), "sqeuclid", normalize=False)
+    out = P.optimize(P.default_parameters(1e-4, iteration_limit=3), pts, storage=_lib.STORAGE_F64)
+    assert out.iteration_count == 2 and len(out.iteration_stats) == 3
+    ci = out.iteration_stats[2].convergence_information[0]
+    norm, obj = third_evaluation(pts, block=256)
+    print(f"||X2||_F {ci.l2_primal_variable:.15g} (host {norm:.15g}), <C,X2> {ci.primal_objective:.15g} (host {obj:.15g})")
+    assert ci.l2_primal_variable == pytest.approx(norm, rel=1e-11)
+    assert ci.primal_objective == pytest.approx(obj, rel=1e-11)

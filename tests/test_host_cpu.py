@@ -76,3 +76,17 @@ def test_calibrated_primal_weight_is_stored_with_the_network():
     assert totals[grid.index(w)] == totals.min()
     assert totals[grid.index(w)] < totals[grid.index(1.0)]
     assert int(meta["calibration_seed0"]) == 200000
+
+
+def test_point_third_evaluation_known_answer_matches_oracle():
+    """The host known answer used by the large point-cloud GPU test agrees
+    with the oracle's iteration-3 evaluation on a small problem."""
+    from point_kat import third_evaluation
+    g = np.random.default_rng(3)
+    pts = P.PointCloudProblem(g.normal(size=(130, 2)), g.normal(size=(97, 2)) + [2.0, 0.0],
+                              g.random(130) / 65, g.random(97) / 48.5, "sqeuclid", normalize=False)
+    params = P.default_parameters(1e-4, iteration_limit=3)
+    ref = O.optimize(params, pts.dense(), mode=O.FUSED)
+    norm, obj = third_evaluation(pts, block=40)
+    assert ref["stats"][2]["l2_primal_variable"] == pytest.approx(norm, rel=1e-13)
+    assert ref["stats"][2]["primal_objective"] == pytest.approx(obj, rel=1e-13)
