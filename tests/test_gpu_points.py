@@ -89,17 +89,28 @@ def test_point_first_ten_evaluations_match_oracle():
 def test_point_form_beyond_2_31_elements_known_answer():
     """70000 x 31000 (2.17e9 elements, beyond 32-bit indexing), FP64 storage,
     point-cloud cost: the third evaluation reports ||X2||_F and <C, X2> of
-    the closed-form plan after pass 2 (tests/point_kat.py), to 1e-11."""
+    the closed-form plan after pass 2 (tests/point_kat.py), to 1e-11; the
+    feasibility rounding of the returned plan meets the masses exactly."""
     from point_kat import third_evaluation
     n, m = 70000, 31000
     assert n * m > 2 ** 31
     g = np.random.default_rng(11)
+    a, b = g.random(n), g.random(m)  # balanced masses (each sums to 1), as the rounding requires
     pts = P.PointCloudProblem(g.normal(size=(n, 2)), g.normal(size=(m, 2)) + [2.0, 0.0],
-                              g.random(n) * (2.0 / n), g.random(m) * (2.0 / m), "sqeuclid", normalize=False)
-    out = P.optimize(P.default_parameters(1e-4, iteration_limit=3), pts, storage=_lib.STORAGE_F64)
+                              a / a.sum(), b / b.sum(), "sqeuclid", normalize=False)
+    dev = _lib.DeviceSolver(n, m, storage=_lib.STORAGE_F64)
+    dev.upload_points(pts)
+    out = P.optimize(P.default_parameters(1e-4, iteration_limit=3), pts, storage=_lib.STORAGE_F64, solver=dev)
     assert out.iteration_count == 2 and len(out.iteration_stats) == 3
     ci = out.iteration_stats[2].convergence_information[0]
     norm, obj = third_evaluation(pts, block=256)
     print(f"||X2||_F {ci.l2_primal_variable:.15g} (host {norm:.15g}), <C,X2> {ci.primal_objective:.15g} (host {obj:.15g})")
     assert ci.l2_primal_variable == pytest.approx(norm, rel=1e-11)
     assert ci.primal_objective == pytest.approx(obj, rel=1e-11)
+    # feasibility rounding of the returned plan at the same size: rows and columns sum to the masses
+    Y = dev.round(dev.output_point)
+    dev.close()
+    del out
+    np.testing.assert_allclose(Y.sum(axis=1), pts.source_distribution, rtol=0, atol=1e-13)
+    np.testing.assert_allclose(Y.sum(axis=0), pts.target_distribution, rtol=0, atol=1e-13)
+    assert Y.min() >= -1e-15  # er_i = a_i - rowsum_i can round to a tiny negative, as in the reference
