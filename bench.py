@@ -54,6 +54,16 @@ def measured_traffic(n: int, m: int, form: str):
     return json.loads(block)[form]["traffic_bytes"]
 
 
+def measured_batch_traffic():
+    """HBM bytes per batched step-kernel launch (every instance active) of the
+    512-instance workload, from profiles/hbm_traffic_bstep_kernel_batched_uniform_512.md."""
+    path = os.path.join(REPO, "profiles", "hbm_traffic_bstep_kernel_batched_uniform_512.md")
+    if not os.path.exists(path):
+        return None
+    text = open(path).read()
+    return json.loads(text.split("```json", 1)[1].split("```", 1)[0])["traffic_bytes"]
+
+
 def measure_steps(dev, steps: int, warmup: int):
     from pdot_amd import _lib
     dev.set_start(policy=_lib.POLICY_ADAPTIVE, step_size=1.0, primal_weight=1.0)
@@ -190,7 +200,7 @@ def bench_batched(args) -> int:
            "pass_ms_all_active": tt / steps, "step_kernel_GBs": alg / (ks / steps * 1e-3) / 1e9,
            "bytes_per_pass": alg}
     res["roofline"] = {"bound": "hbm", "achieved": res["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "frac": res["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": None,
+                       "frac": res["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": measured_batch_traffic(),
                        "kernel": "bstep_kernel<float> (every instance active)", "bytes_per_launch": alg,
                        "avg_launch_us": ks / steps * 1e3}
     if not args.no_solve:
