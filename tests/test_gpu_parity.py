@@ -320,3 +320,21 @@ def test_policy_variants_full_solve_results(kind):
     assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL, (kind, out.iteration_count)
     pobj = float(np.sum(prob.cost_matrix * out.primal_solution))
     assert abs(pobj - case["optimum"]) <= 10 * 1e-4 * (1 + abs(case["optimum"]))
+
+
+@pytest.mark.parametrize("n,m", [(2, 7), (3, 1), (5, 130), (67, 3), (129, 2)])
+def test_fp32_storage_ragged_row_vectors(n, m):
+    """FP32 storage packs four rows per lane: row counts that leave a lane's
+    vector partly past n (and single columns) solve to the oracle's FP64
+    objective, the returned plan's KKT recomputed from scratch."""
+    prob = instances.uniform_sqeuclid(n, m, seed=n * 1000 + m)
+    tol = 1e-6
+    out = P.optimize(P.default_parameters(tol), prob, storage=_lib.STORAGE_F32)
+    ref = O.optimize(P.default_parameters(1e-9), prob, mode=O.FUSED)
+    assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+    assert ref["termination_reason"] == int(P.TerminationReason.TERMINATION_REASON_OPTIMAL)
+    pstar = ref["stats"][-1]["primal_objective"]
+    kkt, pobj = recomputed_kkt(prob, out.primal_solution, out.dual_source_solution, out.dual_target_solution)
+    cscale = np.abs(prob.cost_matrix).max()
+    assert kkt < 1.01 * tol + 1e-6 * cscale
+    assert abs(pobj - pstar) <= 10 * tol * (1 + abs(pstar)) + 1e-6 * cscale
