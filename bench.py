@@ -33,12 +33,15 @@ METRIC = "time-to-1e-4 rel KKT at 10k×10k, FP32/learned vs FP64; iters/s; HBM G
 PEAK_HBM_GBS = 8000.0
 
 
-def algorithmic_bytes_per_step(n: int, m: int, elem: int) -> int:
-    """Bytes one PDHG iteration must move: read C, X, X_avg and write X+,
-    X_avg+ (5 n*m element passes), plus the row/column vectors (u, v, r, c
-    and their averages read and written, a and b read: about 10 doubles per
-    row and per column)."""
-    return 5 * n * m * elem + 10 * 8 * (n + m)
+def algorithmic_bytes_per_step(n: int, m: int, elem: int, matrix_streams: int = 4) -> int:
+    """Bytes one PDHG iteration moves on average, plus the row/column vectors
+    (u, v, r, c and their averages read and written, a and b read: about 10
+    doubles per row and per column).  A pass reads C and X and writes X+ (3
+    n*m element streams); the running average is read and written (2 more)
+    by every second pass only, which also applies the update the pass before
+    it deferred: 4 streams per iteration on average (3 without a stored C,
+    the point-cloud form).  With deferred averaging off, 5."""
+    return matrix_streams * n * m * elem + 10 * 8 * (n + m)
 
 
 def measured_traffic(n: int, m: int, form: str):
@@ -151,7 +154,7 @@ def bench_l1_mixture(args) -> int:
     dev.upload_points(pts)
     up = time.time() - t0
     ks, tt, st = measure_steps(dev, args.steps, args.warmup)
-    alg = 4 * n * m * 4 + 80 * (n + m)
+    alg = algorithmic_bytes_per_step(n, m, 4, matrix_streams=3)
     res = {"upload_sec": up, "ms_per_step": tt / args.steps, "step_kernel_GBs": alg / (ks / args.steps * 1e-3) / 1e9,
            "device_GB": dev.device_bytes() / 1e9}
     if not args.no_solve:
@@ -368,7 +371,7 @@ def main() -> int:
         dev = _lib.DeviceSolver(n, m, storage=storage)
         dev.upload_points(pts)
         ks, tt, st = measure_steps(dev, args.steps, args.warmup)
-        alg = 4 * n * m * elem + 80 * (n + m)
+        alg = algorithmic_bytes_per_step(n, m, elem, matrix_streams=3)
         k_avg = ks / args.steps * 1e-3
         points[name] = {"ms_per_step": tt / args.steps, "iters_per_s": args.steps / (tt * 1e-3),
                         "step_kernel_us": k_avg * 1e6, "step_kernel_GBs": alg / k_avg / 1e9, "bytes_per_step": alg,

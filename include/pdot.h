@@ -69,7 +69,9 @@ typedef struct pdot_solver_state {
     int32_t pending;              /* internal: a speculative pass awaits commit */
     int32_t spare0, spare1;       /* internal: slots written by the pending pass */
     int64_t step_budget;          /* internal (batches): passes left in the current run */
-    int64_t reserved;
+    int32_t defer_ok;             /* internal: passes may defer the average's update (single problems) */
+    int32_t avg_pending;          /* internal: the average lacks the current point's contribution */
+    double pend_nw, pend_aw;      /* internal: the averaging weights of that contribution */
 } pdot_solver_state;
 
 /* Reductions of one point (X, u, v, r, c, G) with G = (C - u 1^T) - 1 v^T,
@@ -198,6 +200,14 @@ int pdot_time_steps(pdot_ctx* ctx, int64_t count, double* step_kernel_ms, double
 /* Use a captured hipGraph for pdot_run_steps sequences (1) or plain stream
  * launches (0, the default). */
 int pdot_use_graphs(pdot_ctx* ctx, int32_t enable);
+
+/* Deferred averaging (on by default): an ordinary pass may leave the running
+ * average's update to the next pass, which applies it in registers, so that
+ * every second pass streams three n x m arrays instead of five.  Results are
+ * bit for bit those of updating the average every pass.  enable = 0 turns it
+ * off (for comparison); a lagging contribution is written out first.  No
+ * counterpart in the reference (a data-movement choice of this library). */
+int pdot_set_defer(pdot_ctx* ctx, int32_t enable);
 
 /* Launch geometry of the streaming step kernel: rows per block, column
  * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the

@@ -114,14 +114,6 @@ __device__ __forceinline__ void slot_ptrs(const BParams& p, const BInst& in, con
     a.avg_empty = avg_empty;
 }
 
-__device__ __forceinline__ void avg_weights(const pdot_solver_state& s, double* nw, double* aw) {
-    *nw = 1.0;
-    *aw = 1.0;
-    if (s.sum_weights > 0.0) {
-        *aw = s.sum_weights / (s.sum_weights + s.step_size);
-        *nw = s.step_size / s.sum_weights;
-    }
-}
 
 // Persistent step kernel over the tile list.
 template <typename T, bool EVAL>
@@ -159,6 +151,9 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             avg_weights(sst, &a.nw, &a.aw);
             a.gzero = sst.gzero != 0;
             a.do_step = do_step;
+            a.defer = false;  // batched states never defer the average's update (defer_ok = 0)
+            a.pend = false;
+            a.pnw = a.paw = 1.0;
             double dd = 0.0, ev[kMatStats];
             step_tile<T, EVAL, kBUnroll, kBWaves, kBPipe>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
@@ -207,6 +202,9 @@ __device__ __forceinline__ ChunkArgs chunk_args(const BParams& p, const BInst& i
     avg_weights(s, &c.nw, &c.aw);
     c.avg_empty = avg_empty;
     c.do_step = do_step;
+    c.defer = false;  // the batch updates every average every pass
+    c.pend = false;
+    c.pnw = c.paw = 1.0;
     return c;
 }
 

@@ -190,6 +190,27 @@ __device__ __forceinline__ void choose_spares(pdot_solver_state& st) {
     }
 }
 
+// Averaging weights of a pass from state st (saddle_point_gpu.jl:129-130,
+// weight = the pass's step size): new = (avg + x*nw)*aw; nw = aw = 1 for the
+// first point after a reset, so that the average equals it bit for bit.
+__device__ __forceinline__ void avg_weights(const pdot_solver_state& st, double* nw, double* aw) {
+    *nw = 1.0;
+    *aw = 1.0;
+    if (st.sum_weights > 0.0) {
+        *aw = st.sum_weights / (st.sum_weights + st.step_size);
+        *nw = st.step_size / st.sum_weights;
+    }
+}
+
+// Whether an ordinary pass from state st defers the average's update: only
+// where allowed (single problems), never for an evaluation's pass, and only
+// when no contribution is already waiting and the average is not empty (the
+// first pass after a reset writes no average anyway: it aliases the point).
+// The step kernel and the commit decide alike from the same state.
+__device__ __forceinline__ bool pass_defers(const pdot_solver_state& st, bool eval) {
+    return st.defer_ok != 0 && !eval && st.avg_pending == 0 && st.avg_count > 0;
+}
+
 // Commit the pending pass: take_step!'s acceptance test and step-size
 // recurrence (src/solver_gpu.jl:296-366), its replay of a rejected pass (the
 // replay recomputes an identical pass, so only the scalars change), and the
@@ -199,6 +220,7 @@ __device__ __forceinline__ void choose_spares(pdot_solver_state& st) {
 // numerical_error and is not applied (:338-344); a NaN step-size limit, on
 // which the reference loops forever, is reported the same way.
 __device__ inline void controller_commit(pdot_solver_state& st, double dot, double yy, double dx2) {
+    const bool was_eval = st.pending != 0;
     st.pending = 0;
     if (st.step_budget > 0) st.step_budget -= 1;
     const double eta = st.step_size;
@@ -256,7 +278,22 @@ __device__ inline void controller_commit(pdot_solver_state& st, double dot, doub
         commit = done;
     }
     if (commit) {
-        st.avg_slot = (st.avg_count == 0) ? st.spare0 : st.spare1;
+        if (pass_defers(st, was_eval)) {
+            // the average stays in its slot; the new point's contribution waits
+            // for the next pass (or for materialize_average) with this pass's
+            // weights, formed from the pre-step eta as the step kernel formed them
+            const double S = st.sum_weights;
+            st.pend_nw = 1.0;
+            st.pend_aw = 1.0;
+            if (S > 0.0) {
+                st.pend_aw = S / (S + eta);
+                st.pend_nw = eta / S;
+            }
+            st.avg_pending = 1;
+        } else {
+            st.avg_slot = (st.avg_count == 0) ? st.spare0 : st.spare1;
+            st.avg_pending = 0;
+        }
         st.cur_slot = st.spare0;
         st.sum_weights += eta;
         st.avg_count += 1;
@@ -393,6 +430,13 @@ struct TileArgs {
     int64_t n, ld, row0, row_end, j0, jstep, ncols;
     double eta, omega, nw, aw;
     bool avg_empty, gzero, do_step;
+    // The average's update may lag one pass behind (single problems): a
+    // deferring pass neither reads nor writes the average; the next pass
+    // first applies the lagging contribution of its input point, with the
+    // weights pnw, paw, to the average it loads (in registers, rounded to
+    // the storage type as a store would), then its own.
+    bool defer, pend;
+    double pnw, paw;
 };
 
 // One PDHG pass over a tile (src/solver_gpu.jl:125-174 and 237-264,
@@ -418,6 +462,10 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const bool lane_active = i0 < row_end;
     const bool avg_empty = a.avg_empty;
     const bool do_step = a.do_step;
+    // a deferring pass leaves the average alone (never an evaluation)
+    const bool skip_avg = avg_empty || (!EVAL && a.defer);
+    const bool pend = a.pend && !avg_empty;
+    const double pnw = a.pnw, paw = a.paw;
     const T* __restrict__ C = a.C;
     const T* __restrict__ Xc = a.Xc;
     const T* __restrict__ Xa = a.Xa;
@@ -437,6 +485,10 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
         valid[e] = (i0 + e) < row_end;
         ui[e] = (lane_active && valid[e]) ? a.uc[i0 + e] : 0.0;
         uai[e] = (EVAL && lane_active && valid[e]) ? a.ua[i0 + e] : 0.0;
+        if (EVAL && pend) {  // the average's lagging contribution of u
+            const double t = uai[e] + ui[e] * pnw;
+            uai[e] = t * paw;
+        }
     }
     // point costs: this lane's source coordinates stay in registers
     double xi[VEC][kMaxDim];
@@ -478,7 +530,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                 const int64_t idx = j * ld + i0;
                 B.xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
                 if constexpr (COST == kCostDense) B.cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
-                if (!avg_empty) B.av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
+                if (!skip_avg) B.av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
             }
         }
     };
@@ -499,7 +551,18 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
 #pragma unroll
                     for (int e = 0; e < VEC; ++e) c[e] = point_cost<COST>(xi[e], B.yj[q], a.dim, a.cdiv);
                 }
-                if (!avg_empty) unpack<V, T>(B.av[q], xa);
+                if (!skip_avg) {
+                    unpack<V, T>(B.av[q], xa);
+                    if (pend) {
+                        // the lagging contribution of the input point (the previous
+                        // pass's new point): avg = (avg + x*nw)*aw, stored as T
+#pragma unroll
+                        for (int e = 0; e < VEC; ++e) {
+                            const double t = xa[e] + x[e] * pnw;
+                            xa[e] = (double)(T)(t * paw);
+                        }
+                    }
+                }
 #pragma unroll
                 for (int e = 0; e < VEC; ++e) {
                     // G_ij = (C_ij - u_i) - v_j (solver_gpu.jl:249-250); 0 at start (:434)
@@ -512,7 +575,12 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                         ev[3] = nanmax(ev[3], fabs(rc));
                         ev[4] += rc * rc;
                         if (!avg_empty) {
-                            const double ga = valid[e] ? (c[e] - uai[e]) - B.vaj[q] : 0.0;
+                            double vaj = B.vaj[q];
+                            if (pend) {  // the average's lagging contribution of v
+                                const double t = vaj + B.vj[q] * pnw;
+                                vaj = t * paw;
+                            }
+                            const double ga = valid[e] ? (c[e] - uai[e]) - vaj : 0.0;
                             ev[5] += c[e] * xa[e];
                             ev[6] = nanmax(ev[6], fabs(xa[e]));
                             ev[7] += xa[e] * xa[e];
@@ -537,14 +605,14 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                     csl += d;
                     dd += d * d;
                     // avg = (avg + x*nw)*aw (saddle_point_gpu.jl:129-130)
-                    if (!avg_empty) {
+                    if (!skip_avg) {
                         double t = xa[e] + xnew * nw;
                         xan[e] = t * aw;
                     }
                 }
                 if (do_step) {
                     store_vec<T>(Xn + idx, xn);
-                    if (!avg_empty) store_vec<T>(Xan + idx, xan);
+                    if (!skip_avg) store_vec<T>(Xan + idx, xan);
                 }
             }
             const double s = wave_sum(csl);
@@ -619,6 +687,11 @@ struct ChunkArgs {
     int64_t sq_lo, sq_hi;
     double eta, omega, nw, aw;
     bool avg_empty, do_step;
+    // the averages lag as the matrix average does: a deferring pass writes
+    // none; a pass with a lagging contribution applies it (weights pnw, paw)
+    // to the average it loads before its own
+    bool defer, pend;
+    double pnw, paw;
 };
 
 // Finishes one pass for a chunk (src/solver_gpu.jl:172-173, 177-234,
@@ -649,6 +722,17 @@ __device__ __forceinline__ void finish_chunk(const ChunkArgs& c, int lane, doubl
         const double q = c.q[k];
         const double s_k = c.s_cur[k];
         const double y_k = c.y_cur[k];
+        double sa = 0.0, ya = 0.0;
+        if (!avg_empty) {
+            sa = c.s_avg[k];
+            ya = c.y_avg[k];
+            if (c.pend) {  // the input point's lagging contribution
+                const double t1 = sa + s_k * c.pnw;
+                sa = t1 * c.paw;
+                const double t2 = ya + y_k * c.pnw;
+                ya = t2 * c.paw;
+            }
+        }
         if (do_step) {
             const double sn = sum + s_k;
             const double yn = y_k + eta * omega * (q - 2 * sn + s_k);
@@ -657,10 +741,10 @@ __device__ __forceinline__ void finish_chunk(const ChunkArgs& c, int lane, doubl
             yy = dy * dy;
             c.s_new[k] = sn;
             c.y_new[k] = yn;
-            if (!avg_empty) {
-                const double t1 = c.s_avg[k] + sn * nw;
+            if (!avg_empty && !(c.defer && !EVAL)) {
+                const double t1 = sa + sn * nw;
                 c.s_anew[k] = t1 * aw;
-                const double t2 = c.y_avg[k] + yn * nw;
+                const double t2 = ya + yn * nw;
                 c.y_anew[k] = t2 * aw;
             }
         }
@@ -673,7 +757,6 @@ __device__ __forceinline__ void finish_chunk(const ChunkArgs& c, int lane, doubl
             ev[4] = rows ? y_k * y_k : 0.0;
             ev[5] = y_k * y_k;
             if (!avg_empty) {
-                const double sa = c.s_avg[k], ya = c.y_avg[k];
                 const double ta = q - sa;
                 ev[6] = fabs(ta);
                 ev[7] = ta * ta;

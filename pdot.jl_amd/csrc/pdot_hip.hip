@@ -150,6 +150,10 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     }
     a.gzero = sst.gzero != 0;
     a.do_step = do_step;
+    a.defer = pass_defers(sst, EVAL);
+    a.pend = sst.avg_pending != 0;
+    a.pnw = sst.pend_nw;
+    a.paw = sst.pend_aw;
     double dd = 0.0, ev[kMatStats];
     step_tile<T, EVAL, Shape::kUnroll, WAVES, Shape::kPipe, COST>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
@@ -210,6 +214,10 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     }
     c.avg_empty = avg_empty;
     c.do_step = do_step;
+    c.defer = pass_defers(sst, EVAL);
+    c.pend = sst.avg_pending != 0;
+    c.pnw = sst.pend_nw;
+    c.paw = sst.pend_aw;
     double out3[3], ev[kVecStats];
     reduce_chunk<EVAL, kRedWaves>(c, psum, out3, ev);
     if (tid == 0) {
@@ -276,6 +284,35 @@ __global__ __launch_bounds__(kWave) void commit_kernel(pdot_solver_state* st, co
     if (!sst.pending) return;
     reduce_and_commit(&sst, redpart, nred);
     if (threadIdx.x == 0) *st = sst;
+}
+
+// Applies the average's lagging contribution in place: avg = (avg + cur*nw)*aw
+// elementwise, stored as T, exactly as the next pass would have applied it
+// in registers (saddle_point_gpu.jl:129-130).
+template <typename T>
+__global__ __launch_bounds__(256) void materialize_kernel(T* __restrict__ avg, const T* __restrict__ cur, int64_t total,
+                                                          double nw, double aw) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+        const double t = (double)avg[k] + (double)cur[k] * nw;
+        avg[k] = (T)(t * aw);
+    }
+}
+
+// The same for the four vector averages (u, r of length n; v, c of length m).
+struct VecAverages {
+    double* avg[4];
+    const double* cur[4];
+    int64_t len[4];
+};
+
+__global__ __launch_bounds__(256) void materialize_vectors_kernel(VecAverages p, double nw, double aw) {
+    const int w = blockIdx.y;
+    double* __restrict__ avg = p.avg[w];
+    const double* __restrict__ cur = p.cur[w];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < p.len[w]; k += (int64_t)gridDim.x * blockDim.x) {
+        const double t = avg[k] + cur[k] * nw;
+        avg[k] = t * aw;
+    }
 }
 
 constexpr int kIngestBlocks = 1024;  // blocks per ingest chunk
@@ -403,6 +440,7 @@ struct pdot_ctx {
     int32_t interleave = 1;
     bool uploaded = false, started = false;
     bool use_graph = false;
+    bool defer = true;  // ordinary passes alternate deferring / applying the average's update
     std::map<int64_t, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> events;
     int64_t bytes = 0;
@@ -632,6 +670,33 @@ int write_state(pdot_ctx* ctx, const pdot_solver_state* in) {
     HIPCHK(ctx, hipMemcpyAsync(ctx->stA, ctx->h_state, sizeof(pdot_solver_state), hipMemcpyHostToDevice, ctx->stream));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     return PDOT_OK;
+}
+
+// Writes the average's lagging contribution (if any) into the average's slot,
+// so that the slot holds the average itself; st is updated and written back.
+int materialize_average(pdot_ctx* ctx, pdot_solver_state& st) {
+    if (!st.avg_pending) return PDOT_OK;
+    const int64_t total = ctx->ld * ctx->m;
+    if (ctx->storage == PDOT_STORAGE_F64)
+        hipLaunchKernelGGL(materialize_kernel<double>, dim3(4096), dim3(256), 0, ctx->stream,
+                           static_cast<double*>(ctx->s.X[st.avg_slot]), static_cast<const double*>(ctx->s.X[st.cur_slot]),
+                           total, st.pend_nw, st.pend_aw);
+    else
+        hipLaunchKernelGGL(materialize_kernel<float>, dim3(4096), dim3(256), 0, ctx->stream,
+                           static_cast<float*>(ctx->s.X[st.avg_slot]), static_cast<const float*>(ctx->s.X[st.cur_slot]),
+                           total, st.pend_nw, st.pend_aw);
+    HIPCHK(ctx, hipGetLastError());
+    VecAverages va;
+    double* const* ys[4] = {ctx->s.u, ctx->s.r, ctx->s.v, ctx->s.c};
+    for (int w = 0; w < 4; ++w) {
+        va.avg[w] = ys[w][st.avg_slot];
+        va.cur[w] = ys[w][st.cur_slot];
+        va.len[w] = w < 2 ? ctx->n : ctx->m;
+    }
+    hipLaunchKernelGGL(materialize_vectors_kernel, dim3(64, 4), dim3(256), 0, ctx->stream, va, st.pend_nw, st.pend_aw);
+    HIPCHK(ctx, hipGetLastError());
+    st.avg_pending = 0;
+    return write_state(ctx, &st);
 }
 
 int ensure_stage(pdot_ctx* ctx) {
@@ -980,6 +1045,7 @@ int pdot_set_start(pdot_ctx* ctx, const double* u0, const double* v0, int32_t st
     st.avg_slot = 0;
     st.spare0 = 1;
     st.spare1 = 2;
+    st.defer_ok = ctx->defer ? 1 : 0;
     int rc = write_state(ctx, &st);
     if (rc) return rc;
     ctx->started = true;
@@ -1085,10 +1151,12 @@ int pdot_restart(pdot_ctx* ctx, int32_t to_average, double new_primal_weight) {
     st.pending = 0;
     if (to_average) {
         if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_restart: restart to an empty average");
+        if ((rc = materialize_average(ctx, st))) return rc;
         st.cur_slot = st.avg_slot;
         st.gzero = 0;
     }
     st.avg_count = 0;
+    st.avg_pending = 0;  // the reset average has no lagging contribution
     st.sum_weights = 0.0;
     st.avg_slot = st.cur_slot;
     st.primal_weight = new_primal_weight;
@@ -1106,6 +1174,8 @@ int pdot_set_state(pdot_ctx* ctx, const pdot_solver_state* in) {
     if (in->cur_slot < 0 || in->cur_slot > 3 || in->avg_slot < 0 || in->avg_slot > 3 || in->spare0 < 0 ||
         in->spare0 > 3 || in->spare1 < 0 || in->spare1 > 3)
         return fail(ctx, PDOT_E_ARG, "pdot_set_state: slot index out of range");
+    if ((in->defer_ok != 0 && in->defer_ok != 1) || (in->avg_pending != 0 && in->avg_pending != 1))
+        return fail(ctx, PDOT_E_ARG, "pdot_set_state: defer_ok and avg_pending must be 0 or 1");
     return write_state(ctx, in);
 }
 
@@ -1119,6 +1189,7 @@ int pdot_download(pdot_ctx* ctx, int32_t which, double* X, double* u, double* v)
     if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
     else if (which == PDOT_POINT_AVERAGE) {
         if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_download: the average is empty");
+        if ((rc = materialize_average(ctx, st))) return rc;
         slot = st.avg_slot;
     } else return fail(ctx, PDOT_E_ARG, "pdot_download: unknown point");
     const int64_t n = ctx->n, m = ctx->m, ld = ctx->ld;
@@ -1152,6 +1223,7 @@ int pdot_round(pdot_ctx* ctx, int32_t which, double* X_rounded) {
     if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
     else if (which == PDOT_POINT_AVERAGE) {
         if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_round: the average is empty");
+        if ((rc = materialize_average(ctx, st))) return rc;
         slot = st.avg_slot;
     } else return fail(ctx, PDOT_E_ARG, "pdot_round: unknown point");
     HIPCHK(ctx, hipSetDevice(ctx->device));
@@ -1210,6 +1282,20 @@ int pdot_use_graphs(pdot_ctx* ctx, int32_t enable) {
     if (!ctx) return PDOT_E_ARG;
     ctx->use_graph = enable != 0;
     return PDOT_OK;
+}
+
+int pdot_set_defer(pdot_ctx* ctx, int32_t enable) {
+    if (!ctx) return PDOT_E_ARG;
+    if (enable != 0 && enable != 1) return fail(ctx, PDOT_E_ARG, "pdot_set_defer: enable must be 0 or 1");
+    ctx->defer = enable != 0;
+    if (!ctx->started) return PDOT_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    if (!enable && (rc = materialize_average(ctx, st))) return rc;
+    st.defer_ok = enable;
+    return write_state(ctx, &st);
 }
 
 int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint, int32_t interleave) {

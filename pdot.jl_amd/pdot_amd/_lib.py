@@ -57,7 +57,10 @@ class SolverState(ctypes.Structure):
         ("spare0", ctypes.c_int32),
         ("spare1", ctypes.c_int32),
         ("step_budget", ctypes.c_int64),
-        ("reserved", ctypes.c_int64),
+        ("defer_ok", ctypes.c_int32),
+        ("avg_pending", ctypes.c_int32),
+        ("pend_nw", ctypes.c_double),
+        ("pend_aw", ctypes.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -107,6 +110,7 @@ SIGNATURES = {
     "pdot_download": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p]),
     "pdot_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
     "pdot_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
@@ -300,6 +304,10 @@ class DeviceSolver:
 
     def use_graphs(self, enable: bool) -> None:
         self._check(self.lib.pdot_use_graphs(self.h, int(bool(enable))), "pdot_use_graphs")
+
+    def set_defer(self, enable: bool) -> None:
+        """Deferred averaging on (default) or off; results are identical either way."""
+        self._check(self.lib.pdot_set_defer(self.h, int(bool(enable))), "pdot_set_defer")
 
     def set_launch(self, col_blocks_hint: int = 0, interleave: bool = True) -> None:
         self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint), int(bool(interleave))), "pdot_set_launch")

@@ -29,8 +29,9 @@ def test_binding_covers_header():
 
 
 def test_struct_sizes_match_header_layout():
-    # pdot_solver_state: 4 doubles, 4 int64, 8 doubles, 8 int32, 2 int64
-    assert ctypes.sizeof(_lib.SolverState) == 4 * 8 + 4 * 8 + 8 * 8 + 8 * 4 + 2 * 8
+    # pdot_solver_state: 4 doubles, 4 int64, 8 doubles, 8 int32, 1 int64, 2 int32, 2 doubles
+    assert ctypes.sizeof(_lib.SolverState) == 4 * 8 + 4 * 8 + 8 * 8 + 8 * 4 + 8 + 2 * 4 + 2 * 8
+    assert _lib.SolverState.pend_aw.offset == 184
     assert ctypes.sizeof(_lib.PointStats) == 11 * 8
 
 

@@ -1,0 +1,143 @@
+"""Deferred averaging (on by default for single problems): an ordinary pass
+may leave its contribution to the running average (X, u, v, r, c) to the
+next pass, which applies it to the average it loads, rounded to the storage
+type as a store would round it, before adding its own.  Every second pass
+then neither reads nor writes the average matrix.  Each element's arithmetic
+is unchanged, so every result must equal the one with deferral off bit for
+bit: whole solves (iterations, restarts, evaluations, plan and duals), and a
+stepping sequence that reads the average while a contribution lags
+(downloads, evaluations, rounding, restarts, deferral switched mid-solve)."""
+import numpy as np
+import pytest
+
+import pdot_amd as P
+from pdot_amd import _lib, instances
+
+pytestmark = pytest.mark.gpu
+
+# the solver state's numeric fields; the slot indices and the deferral
+# bookkeeping legitimately differ between the two paths
+_STATE_FIELDS = ("step_size", "primal_weight", "cumulative_kkt_passes", "sum_weights", "total_number_iterations",
+                 "avg_count", "take_step_calls", "error_call", "last_interaction", "last_movement", "last_step_limit",
+                 "last_dx2", "last_dot", "last_dy2", "numerical_error", "gzero", "pending")
+
+
+def _same_state(s1, s2):
+    for k in _STATE_FIELDS:
+        assert getattr(s1, k) == getattr(s2, k), k
+
+
+def _same_arrays(xs, ys):
+    for x, y in zip(xs, ys):
+        assert np.array_equal(x, y)
+
+
+def _device(prob, storage, defer, points):
+    n = len(prob.source_distribution)
+    m = len(prob.target_distribution)
+    dev = _lib.DeviceSolver(n, m, storage=storage)
+    if points:
+        dev.upload_points(prob)
+    else:
+        dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+    dev.set_defer(defer)
+    return dev
+
+
+CASES = [("dense", _lib.STORAGE_F64, lambda: instances.gaussian_shift(300, 257, seed=11)),
+         ("dense", _lib.STORAGE_F32, lambda: instances.gaussian_shift(300, 257, seed=11)),
+         ("dense", _lib.STORAGE_F32, lambda: instances.uniform_sqeuclid(61, 1003, seed=12)),
+         ("points", _lib.STORAGE_F64, lambda: instances.l1_mixture_3d_points(400, 450, seed=13)),
+         ("points", _lib.STORAGE_F32, lambda: instances.gaussian_shift_points(517, 260, seed=14))]
+
+
+@pytest.mark.parametrize("form,storage,make", CASES)
+def test_deferred_solve_equals_eager_bitwise(form, storage, make):
+    prob = make()
+    outs = []
+    for defer in (True, False):
+        dev = _device(prob, storage, defer, form == "points")
+        out = P.optimize(P.default_parameters(1e-4), prob, storage=storage, solver=dev)
+        avg = dev.download(_lib.POINT_AVERAGE)
+        cur = dev.download(_lib.POINT_CURRENT)
+        Y = dev.round(_lib.POINT_AVERAGE)
+        dev.close()
+        outs.append((out, avg, cur, Y))
+    (a, avg_a, cur_a, Ya), (b, avg_b, cur_b, Yb) = outs
+    assert a.termination_reason == b.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+    assert a.iteration_count == b.iteration_count > 100
+    assert len(a.iteration_stats) == len(b.iteration_stats)
+    for sa, sb in zip(a.iteration_stats, b.iteration_stats):
+        assert sa.restart_used == sb.restart_used
+        assert sa.step_size == sb.step_size and sa.primal_weight == sb.primal_weight
+        for ca, cb in zip(sa.convergence_information, sb.convergence_information):
+            assert ca.primal_objective == cb.primal_objective and ca.dual_objective == cb.dual_objective
+            assert ca.l2_primal_residual == cb.l2_primal_residual and ca.l2_dual_residual == cb.l2_dual_residual
+    _same_arrays((a.primal_solution, a.dual_source_solution, a.dual_target_solution),
+                 (b.primal_solution, b.dual_source_solution, b.dual_target_solution))
+    _same_arrays(avg_a + cur_a + (Ya,), avg_b + cur_b + (Yb,))
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_lagging_average_reads_equal_eager_updates(storage):
+    prob = instances.gaussian_shift(200, 170, seed=7)
+    devs = [_device(prob, storage, defer, False) for defer in (True, False)]
+
+    def both(f):
+        return [f(d) for d in devs]
+
+    def steps(k):
+        _same_state(*both(lambda d: d.run_steps(k)))
+
+    def lag():  # single passes until the deferring solver holds a lagging contribution
+        for _ in range(2):
+            if devs[0].get_state().avg_pending:
+                break
+            steps(1)
+        assert devs[0].get_state().avg_pending == 1 and devs[1].get_state().avg_pending == 0
+
+    def check_average(average=True):
+        if average:
+            _same_arrays(*both(lambda d: d.download(_lib.POINT_AVERAGE)))
+        _same_arrays(*both(lambda d: d.download(_lib.POINT_CURRENT)))
+        _same_state(*both(lambda d: d.get_state()))
+
+    def check_eval():
+        evs = both(lambda d: d.eval())
+        assert evs[0][0].as_dict() == evs[1][0].as_dict() and evs[0][1].as_dict() == evs[1][1].as_dict()
+        _same_state(evs[0][2], evs[1][2])
+
+    both(lambda d: d.set_start(primal_weight=1.5))
+    steps(4)
+    lag()
+    check_average()  # downloads materialize the lagging contribution
+    steps(5)
+    lag()
+    check_eval()  # an evaluation pass applies the lagging contribution
+    steps(4)  # commits the evaluation's pass first
+    lag()
+    _same_arrays(*both(lambda d: [d.round(_lib.POINT_AVERAGE)]))
+    steps(7)
+    lag()
+    both(lambda d: d.restart(True, 2.25))  # restart to an average whose last contribution lags
+    check_average(average=False)  # the restart empties the average
+    steps(1)
+    check_average()
+    steps(9)
+    lag()
+    devs[0].set_defer(False)  # switched off with a contribution lagging, then on again
+    steps(3)
+    devs[0].set_defer(True)
+    steps(11)
+    lag()
+    check_eval()
+    both(lambda d: d.discard())  # a discarded evaluation leaves the lag in place
+    assert devs[0].get_state().avg_pending == 1
+    check_average()
+    steps(2)
+    lag()
+    both(lambda d: d.restart(False, 1.0))  # restart to the current point drops the lag
+    check_average(average=False)
+    steps(5)
+    check_average()
+    both(lambda d: d.close())
