@@ -69,7 +69,7 @@ typedef struct pdot_solver_state {
     int32_t pending;              /* internal: a speculative pass awaits commit */
     int32_t spare0, spare1;       /* internal: slots written by the pending pass */
     int64_t step_budget;          /* internal (batches): passes left in the current run */
-    int32_t defer_ok;             /* internal: passes may defer the average's update (single problems) */
+    int32_t defer_ok;             /* internal: passes may defer the average's update */
     int32_t avg_pending;          /* internal: the average lacks the current point's contribution */
     double pend_nw, pend_aw;      /* internal: the averaging weights of that contribution */
 } pdot_solver_state;
@@ -270,6 +270,10 @@ int pdot_batch_set_states(pdot_batch* b, const pdot_solver_state* in);
 int pdot_batch_download(pdot_batch* b, int32_t k, int32_t which, double* X, double* u, double* v);
 int pdot_batch_time_steps(pdot_batch* b, int64_t count, double* step_kernel_ms, double* total_ms);
 int pdot_batch_use_graphs(pdot_batch* b, int32_t enable);
+/* Deferred averaging for every instance of the batch, as pdot_set_defer does
+ * for a single problem (on by default; bit-identical results).  No
+ * counterpart in the reference. */
+int pdot_batch_set_defer(pdot_batch* b, int32_t enable);
 /* Launch shape of the persistent step kernel (tuned by pdot_amd.batch.autotune):
  * columns per tile and the grid as a multiple of the resident block count. */
 int pdot_batch_set_launch(pdot_batch* b, int32_t cols_per_tile, int32_t grid_mult);

@@ -151,9 +151,10 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             avg_weights(sst, &a.nw, &a.aw);
             a.gzero = sst.gzero != 0;
             a.do_step = do_step;
-            a.defer = false;  // batched states never defer the average's update (defer_ok = 0)
-            a.pend = false;
-            a.pnw = a.paw = 1.0;
+            a.defer = pass_defers(sst, EVAL);
+            a.pend = sst.avg_pending != 0;
+            a.pnw = sst.pend_nw;
+            a.paw = sst.pend_aw;
             double dd = 0.0, ev[kMatStats];
             step_tile<T, EVAL, kBUnroll, kBWaves, kBPipe>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
 // in state `s`; *lc receives the chunk's index within the instance (rows
 // first, then columns).
 __device__ __forceinline__ ChunkArgs chunk_args(const BParams& p, const BInst& in, const pdot_solver_state& s,
-                                                bool rows, int32_t chunk, bool do_step, int64_t* lc) {
+                                                bool rows, int32_t chunk, bool do_step, bool eval, int64_t* lc) {
     const bool avg_empty = s.avg_count == 0;
     const int cs = s.cur_slot, as = avg_empty ? cs : s.avg_slot;
     const int64_t* yo = rows ? in.u : in.v;
@@ -202,9 +203,10 @@ __device__ __forceinline__ ChunkArgs chunk_args(const BParams& p, const BInst& i
     avg_weights(s, &c.nw, &c.aw);
     c.avg_empty = avg_empty;
     c.do_step = do_step;
-    c.defer = false;  // the batch updates every average every pass
-    c.pend = false;
-    c.pnw = c.paw = 1.0;
+    c.defer = pass_defers(s, eval);
+    c.pend = s.avg_pending != 0;
+    c.pnw = s.pend_nw;
+    c.paw = s.pend_aw;
     return c;
 }
 
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
         const bool do_step = !sst.numerical_error && (EVAL || sst.step_budget > 0);
         if (do_step || EVAL) {
             int64_t lc;
-            const ChunkArgs c = chunk_args(p, sinst, sst, sch.rows != 0, sch.chunk, do_step, &lc);
+            const ChunkArgs c = chunk_args(p, sinst, sst, sch.rows != 0, sch.chunk, do_step, EVAL, &lc);
             double out3[3], ev[kVecStats];
             reduce_chunk<EVAL, kBRedWaves>(c, psum, out3, ev);
             if (tid == 0) {
@@ -258,7 +260,7 @@ __global__ __launch_bounds__(kBRedBlock) void breduce_kernel(BParams p) {
         const bool do_step = !gs.numerical_error && (EVAL || gs.step_budget > 0);
         if (!do_step && !EVAL) continue;
         int64_t lc;
-        const ChunkArgs c = chunk_args(p, in, gs, ch.rows != 0, ch.chunk, do_step, &lc);
+        const ChunkArgs c = chunk_args(p, in, gs, ch.rows != 0, ch.chunk, do_step, EVAL, &lc);
         double out3[3], ev[kVecStats];
         reduce_chunk_wave<EVAL>(c, out3, ev);
         if (lane == 0) {
@@ -395,6 +397,7 @@ struct pdot_batch {
     bool started = false;
     bool layout_ok = false;  // partial arenas and work lists match the active set
     bool use_graph = false;
+    bool defer = true;  // ordinary passes alternate deferring / applying the averages' update
     std::map<int64_t, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> events;
     int64_t bytes = 0;
@@ -649,6 +652,36 @@ void bchoose_spares(pdot_solver_state& st) {
     }
 }
 
+// Writes instance k's lagging average contribution (if any) into its average
+// slots, so that they hold the average itself; updates b->h_st[k] only (the
+// caller writes the states back).
+int bmaterialize(pdot_batch* b, int32_t k) {
+    pdot_solver_state& st = b->h_st[k];
+    if (!st.avg_pending) return PDOT_OK;
+    const BInst& in = b->inst[k];
+    const int64_t total = in.ld * in.m;
+    if (b->storage == PDOT_STORAGE_F64)
+        hipLaunchKernelGGL(materialize_kernel<double>, dim3(256), dim3(256), 0, b->stream,
+                           static_cast<double*>(b->mat) + in.X[st.avg_slot],
+                           static_cast<const double*>(b->mat) + in.X[st.cur_slot], total, st.pend_nw, st.pend_aw);
+    else
+        hipLaunchKernelGGL(materialize_kernel<float>, dim3(256), dim3(256), 0, b->stream,
+                           static_cast<float*>(b->mat) + in.X[st.avg_slot],
+                           static_cast<const float*>(b->mat) + in.X[st.cur_slot], total, st.pend_nw, st.pend_aw);
+    BHIPCHK(b, hipGetLastError());
+    VecAverages va;
+    const int64_t* offs[4] = {in.u, in.r, in.v, in.c};
+    for (int w = 0; w < 4; ++w) {
+        va.avg[w] = b->vec + offs[w][st.avg_slot];
+        va.cur[w] = b->vec + offs[w][st.cur_slot];
+        va.len[w] = w < 2 ? in.n : in.m;
+    }
+    hipLaunchKernelGGL(materialize_vectors_kernel, dim3(8, 4), dim3(256), 0, b->stream, va, st.pend_nw, st.pend_aw);
+    BHIPCHK(b, hipGetLastError());
+    st.avg_pending = 0;
+    return PDOT_OK;
+}
+
 int bensure_stage(pdot_batch* b, int64_t elems) {
     if (b->stage && b->stage_elems >= elems) return PDOT_OK;
     if (b->stage) {
@@ -801,6 +834,7 @@ int pdot_batch_set_start(pdot_batch* b, int32_t step_policy, double reduction_ex
         st.avg_slot = 0;
         st.spare0 = 1;
         st.spare1 = 2;
+        st.defer_ok = b->defer ? 1 : 0;
     }
     int rc = bwrite_states(b);
     if (rc) return rc;
@@ -935,15 +969,21 @@ int pdot_batch_restart(pdot_batch* b, const int32_t* action, const double* new_p
     for (int32_t k = 0; k < b->count; ++k) {
         if (!action[k]) continue;
         pdot_solver_state& st = b->h_st[k];
+        if (action[k] != 1 && action[k] != 2) return bfail(b, PDOT_E_ARG, "pdot_batch_restart: action must be 0, 1 or 2");
+        if (action[k] == 2 && st.avg_count == 0)
+            return bfail(b, PDOT_E_STATE, "pdot_batch_restart: restart to an empty average");
+    }
+    for (int32_t k = 0; k < b->count; ++k) {
+        if (!action[k]) continue;
+        pdot_solver_state& st = b->h_st[k];
         st.pending = 0;
         if (action[k] == 2) {
-            if (st.avg_count == 0) return bfail(b, PDOT_E_STATE, "pdot_batch_restart: restart to an empty average");
+            if ((rc = bmaterialize(b, k))) return rc;
             st.cur_slot = st.avg_slot;
             st.gzero = 0;
-        } else if (action[k] != 1) {
-            return bfail(b, PDOT_E_ARG, "pdot_batch_restart: action must be 0, 1 or 2");
         }
         st.avg_count = 0;
+        st.avg_pending = 0;  // the reset average has no lagging contribution
         st.sum_weights = 0.0;
         st.avg_slot = st.cur_slot;
         st.primal_weight = new_primal_weight[k];
@@ -971,6 +1011,10 @@ int pdot_batch_download(pdot_batch* b, int32_t k, int32_t which, double* X, doub
     if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
     else if (which == PDOT_POINT_AVERAGE) {
         if (st.avg_count == 0) return bfail(b, PDOT_E_STATE, "pdot_batch_download: the average is empty");
+        if (st.avg_pending) {
+            if ((rc = bmaterialize(b, k))) return rc;
+            if ((rc = bwrite_states(b))) return rc;
+        }
         slot = st.avg_slot;
     } else return bfail(b, PDOT_E_ARG, "pdot_batch_download: unknown point");
     const BInst& in = b->inst[k];
@@ -1037,6 +1081,21 @@ int pdot_batch_use_graphs(pdot_batch* b, int32_t enable) {
     if (!b) return PDOT_E_ARG;
     b->use_graph = enable != 0;
     return PDOT_OK;
+}
+
+int pdot_batch_set_defer(pdot_batch* b, int32_t enable) {
+    if (!b) return PDOT_E_ARG;
+    if (enable != 0 && enable != 1) return bfail(b, PDOT_E_ARG, "pdot_batch_set_defer: enable must be 0 or 1");
+    b->defer = enable != 0;
+    if (!b->started) return PDOT_OK;
+    BHIPCHK(b, hipSetDevice(b->device));
+    int rc = bread_states(b);
+    if (rc) return rc;
+    for (int32_t k = 0; k < b->count; ++k) {
+        if (!enable && (rc = bmaterialize(b, k))) return rc;
+        b->h_st[k].defer_ok = enable;
+    }
+    return bwrite_states(b);
 }
 
 int pdot_batch_set_launch(pdot_batch* b, int32_t cols_per_tile, int32_t grid_mult) {

@@ -203,12 +203,41 @@ __device__ __forceinline__ void avg_weights(const pdot_solver_state& st, double*
 }
 
 // Whether an ordinary pass from state st defers the average's update: only
-// where allowed (single problems), never for an evaluation's pass, and only
+// where allowed (defer_ok), never for an evaluation's pass, and only
 // when no contribution is already waiting and the average is not empty (the
 // first pass after a reset writes no average anyway: it aliases the point).
 // The step kernel and the commit decide alike from the same state.
 __device__ __forceinline__ bool pass_defers(const pdot_solver_state& st, bool eval) {
     return st.defer_ok != 0 && !eval && st.avg_pending == 0 && st.avg_count > 0;
+}
+
+// Applies the average's lagging contribution in place: avg = (avg + cur*nw)*aw
+// elementwise, stored as T, exactly as the next pass would have applied it
+// in registers (saddle_point_gpu.jl:129-130).
+template <typename T>
+__global__ __launch_bounds__(256) void materialize_kernel(T* __restrict__ avg, const T* __restrict__ cur, int64_t total,
+                                                          double nw, double aw) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
+        const double t = (double)avg[k] + (double)cur[k] * nw;
+        avg[k] = (T)(t * aw);
+    }
+}
+
+// The same for the four vector averages (u, r of length n; v, c of length m).
+struct VecAverages {
+    double* avg[4];
+    const double* cur[4];
+    int64_t len[4];
+};
+
+__global__ __launch_bounds__(256) void materialize_vectors_kernel(VecAverages p, double nw, double aw) {
+    const int w = blockIdx.y;
+    double* __restrict__ avg = p.avg[w];
+    const double* __restrict__ cur = p.cur[w];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < p.len[w]; k += (int64_t)gridDim.x * blockDim.x) {
+        const double t = avg[k] + cur[k] * nw;
+        avg[k] = t * aw;
+    }
 }
 
 // Commit the pending pass: take_step!'s acceptance test and step-size
@@ -430,7 +459,7 @@ struct TileArgs {
     int64_t n, ld, row0, row_end, j0, jstep, ncols;
     double eta, omega, nw, aw;
     bool avg_empty, gzero, do_step;
-    // The average's update may lag one pass behind (single problems): a
+    // The average's update may lag one pass behind: a
     // deferring pass neither reads nor writes the average; the next pass
     // first applies the lagging contribution of its input point, with the
     // weights pnw, paw, to the average it loads (in registers, rounded to

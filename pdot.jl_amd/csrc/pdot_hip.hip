@@ -286,35 +286,6 @@ __global__ __launch_bounds__(kWave) void commit_kernel(pdot_solver_state* st, co
     if (threadIdx.x == 0) *st = sst;
 }
 
-// Applies the average's lagging contribution in place: avg = (avg + cur*nw)*aw
-// elementwise, stored as T, exactly as the next pass would have applied it
-// in registers (saddle_point_gpu.jl:129-130).
-template <typename T>
-__global__ __launch_bounds__(256) void materialize_kernel(T* __restrict__ avg, const T* __restrict__ cur, int64_t total,
-                                                          double nw, double aw) {
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (int64_t)gridDim.x * blockDim.x) {
-        const double t = (double)avg[k] + (double)cur[k] * nw;
-        avg[k] = (T)(t * aw);
-    }
-}
-
-// The same for the four vector averages (u, r of length n; v, c of length m).
-struct VecAverages {
-    double* avg[4];
-    const double* cur[4];
-    int64_t len[4];
-};
-
-__global__ __launch_bounds__(256) void materialize_vectors_kernel(VecAverages p, double nw, double aw) {
-    const int w = blockIdx.y;
-    double* __restrict__ avg = p.avg[w];
-    const double* __restrict__ cur = p.cur[w];
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < p.len[w]; k += (int64_t)gridDim.x * blockDim.x) {
-        const double t = avg[k] + cur[k] * nw;
-        avg[k] = t * aw;
-    }
-}
-
 constexpr int kIngestBlocks = 1024;  // blocks per ingest chunk
 
 // Ingest of one column block of the FP64 cost matrix (n x ncols, source

@@ -137,6 +137,7 @@ SIGNATURES = {
     "pdot_batch_download": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32, _c_double_p, _c_double_p, _c_double_p]),
     "pdot_batch_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
     "pdot_batch_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_batch_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_batch_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_batch_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_batch_device_bytes": (ctypes.c_int64, [_ctx_p]),
@@ -440,6 +441,10 @@ class BatchSolver:
 
     def use_graphs(self, enable: bool) -> None:
         self._check(self.lib.pdot_batch_use_graphs(self.h, int(bool(enable))), "pdot_batch_use_graphs")
+
+    def set_defer(self, enable: bool) -> None:
+        """Deferred averaging on (default) or off; results are identical either way."""
+        self._check(self.lib.pdot_batch_set_defer(self.h, int(bool(enable))), "pdot_batch_set_defer")
 
     def set_launch(self, cols_per_tile: int, grid_mult: int = 1) -> None:
         self._check(self.lib.pdot_batch_set_launch(self.h, int(cols_per_tile), int(grid_mult)), "pdot_batch_set_launch")

@@ -141,3 +141,96 @@ def test_lagging_average_reads_equal_eager_updates(storage):
     steps(5)
     check_average()
     both(lambda d: d.close())
+
+
+def _batch(probs, storage, defer):
+    b = _lib.BatchSolver([p.num_row for p in probs], [p.num_col for p in probs], storage=storage)
+    for k, p in enumerate(probs):
+        b.upload(k, p.cost_matrix, p.source_distribution, p.target_distribution)
+    b.set_defer(defer)
+    return b
+
+
+BATCH_SIZES = [(64, 64), (300, 257), (1, 40), (50, 1), (1100, 70), (130, 1500)]
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_batch_deferred_solve_equals_eager_bitwise(storage):
+    from pdot_amd.batch import optimize_batch
+    probs = [instances.uniform_sqeuclid(n, m, seed=20 + k) for k, (n, m) in enumerate(BATCH_SIZES)]
+    runs = []
+    for defer in (True, False):
+        b = _batch(probs, storage, defer)
+        outs = optimize_batch(P.default_parameters(1e-4), probs, storage=storage, batch=b)
+        states = b.get_states()
+        avgs = [b.download(k, _lib.POINT_AVERAGE) if states[k].avg_count > 0 else None for k in range(len(probs))]
+        b.close()
+        runs.append((outs, avgs))
+    (oa, aa), (ob, ab) = runs
+    for a, b in zip(oa, ob):
+        assert a.termination_reason == b.termination_reason
+        assert a.iteration_count == b.iteration_count
+        assert len(a.iteration_stats) == len(b.iteration_stats)
+        for sa, sb in zip(a.iteration_stats, b.iteration_stats):
+            assert sa.restart_used == sb.restart_used and sa.step_size == sb.step_size
+            ca, cb = sa.convergence_information[0], sb.convergence_information[0]
+            assert ca.primal_objective == cb.primal_objective and ca.l2_dual_residual == cb.l2_dual_residual
+        _same_arrays((a.primal_solution, a.dual_source_solution, a.dual_target_solution),
+                     (b.primal_solution, b.dual_source_solution, b.dual_target_solution))
+    assert sum(a.iteration_count for a in oa) > 500
+    for x, y in zip(aa, ab):
+        assert (x is None) == (y is None)
+        if x is not None:
+            _same_arrays(x, y)
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_batch_lagging_average_reads_equal_eager_updates(storage):
+    probs = [instances.uniform_sqeuclid(n, m, seed=40 + k) for k, (n, m) in enumerate([(70, 90), (257, 33), (5, 300)])]
+    bs = [_batch(probs, storage, defer) for defer in (True, False)]
+    K = len(probs)
+
+    def both(f):
+        return [f(b) for b in bs]
+
+    def same_states(s1, s2):
+        for x, y in zip(s1, s2):
+            _same_state(x, y)
+
+    def check(average=(True,) * K):
+        same_states(*both(lambda b: b.get_states()))
+        for k in range(K):
+            _same_arrays(*both(lambda b: b.download(k, _lib.POINT_CURRENT)))
+            if average[k]:
+                _same_arrays(*both(lambda b: b.download(k, _lib.POINT_AVERAGE)))
+        same_states(*both(lambda b: b.get_states()))
+
+    def check_eval():
+        evs = both(lambda b: b.eval())
+        for k in range(K):
+            assert evs[0][0][k].as_dict() == evs[1][0][k].as_dict()
+            assert evs[0][1][k].as_dict() == evs[1][1][k].as_dict()
+        same_states(evs[0][2], evs[1][2])
+
+    both(lambda b: b.set_start(_lib.POLICY_ADAPTIVE, 2.0, 1.0, 1.0, [1.0, 0.5, 2.0]))
+    same_states(*both(lambda b: b.run_steps([4, 5, 6])))  # instances 0 and 2 end with a lagging contribution
+    assert [s.avg_pending for s in bs[0].get_states()] == [1, 0, 1]
+    assert [s.avg_pending for s in bs[1].get_states()] == [0, 0, 0]
+    check()
+    same_states(*both(lambda b: b.run_steps([3, 4, 3])))
+    check_eval()  # evaluation passes that apply lagging contributions
+    same_states(*both(lambda b: b.run_steps([2, 2, 2])))  # commits the evaluations' passes first
+    same_states(*both(lambda b: b.run_steps([1, 2, 1])))
+    pend = [s.avg_pending for s in bs[0].get_states()]
+    assert any(pend)
+    both(lambda b: b.restart([2, 2, 1], [1.5, 0.75, 3.0]))  # to the average (some lagging) and to the current
+    check(average=(False,) * K)
+    same_states(*both(lambda b: b.run_steps([6, 7, 8])))
+    bs[0].set_defer(False)  # switched off with contributions lagging, then on again
+    same_states(*both(lambda b: b.run_steps([3, 3, 3])))
+    bs[0].set_defer(True)
+    same_states(*both(lambda b: b.run_steps([9, 10, 11])))
+    check_eval()
+    both(lambda b: b.discard([1, 1, 1]))
+    check()
+    both(lambda b: b.close())
