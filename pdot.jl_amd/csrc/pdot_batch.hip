@@ -880,6 +880,14 @@ int pdot_batch_get_states(pdot_batch* b, pdot_solver_state* out) {
 
 int pdot_batch_set_states(pdot_batch* b, const pdot_solver_state* in) {
     if (!b || !in) return PDOT_E_ARG;
+    for (int32_t k = 0; k < b->count; ++k) {
+        const pdot_solver_state& s = in[k];
+        const int32_t slots[4] = {s.cur_slot, s.avg_slot, s.spare0, s.spare1};
+        for (int32_t q : slots)
+            if (q < 0 || q > 3) return bfail(b, PDOT_E_ARG, "pdot_batch_set_states: slot index out of range");
+        if ((s.defer_ok != 0 && s.defer_ok != 1) || (s.avg_pending != 0 && s.avg_pending != 1))
+            return bfail(b, PDOT_E_ARG, "pdot_batch_set_states: defer_ok and avg_pending must be 0 or 1");
+    }
     memcpy(b->h_st, in, sizeof(pdot_solver_state) * b->count);
     return bwrite_states(b);
 }

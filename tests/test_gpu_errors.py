@@ -56,3 +56,40 @@ def test_round_matrix_rejects_bad_shapes():
     prob = instances.uniform_sqeuclid(10, 8, seed=1)
     with pytest.raises(ValueError):
         P.rounding(prob, np.zeros((8, 10)))
+
+
+def test_state_and_defer_arguments_are_validated():
+    prob = instances.uniform_sqeuclid(30, 20, seed=2)
+    dev = _lib.DeviceSolver(30, 20)
+    dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+    dev.set_start()
+    st = dev.run_steps(2)  # the second pass leaves a lagging contribution
+    assert st.avg_pending == 1
+    for field, value, msg in (("cur_slot", 4, "slot index"), ("spare1", -1, "slot index"),
+                              ("avg_pending", 2, "0 or 1"), ("defer_ok", -1, "0 or 1")):
+        bad = _lib.SolverState.from_buffer_copy(st)
+        setattr(bad, field, value)
+        with pytest.raises(_lib.PdotError, match=msg):
+            dev.set_state(bad)
+    with pytest.raises(_lib.PdotError, match="0 or 1"):
+        dev._check(dev.lib.pdot_set_defer(dev.h, 2), "pdot_set_defer")
+    dev.set_state(st)
+    assert dev.get_state().avg_pending == 1
+    dev.close()
+
+    probs = [instances.uniform_sqeuclid(n, m, seed=5 + n) for n, m in ((12, 9), (7, 15))]
+    b = _lib.BatchSolver([p.num_row for p in probs], [p.num_col for p in probs], storage=_lib.STORAGE_F64)
+    for k, p in enumerate(probs):
+        b.upload(k, p.cost_matrix, p.source_distribution, p.target_distribution)
+    b.set_start(_lib.POLICY_ADAPTIVE, 2.0, 1.0, 1.0, 1.0)
+    states = b.run_steps([2, 3])
+    assert [s.avg_pending for s in states] == [1, 0]
+    for field, value, msg in (("avg_slot", 9, "slot index"), ("avg_pending", 3, "0 or 1")):
+        bad = (_lib.SolverState * 2).from_buffer_copy(states)
+        setattr(bad[1], field, value)
+        with pytest.raises(_lib.PdotError, match=msg):
+            b._check(b.lib.pdot_batch_set_states(b.h, bad), "pdot_batch_set_states")
+    with pytest.raises(_lib.PdotError, match="0 or 1"):
+        b._check(b.lib.pdot_batch_set_defer(b.h, 5), "pdot_batch_set_defer")
+    assert [s.avg_pending for s in b.get_states()] == [1, 0]
+    b.close()
