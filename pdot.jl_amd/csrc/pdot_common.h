@@ -422,8 +422,24 @@ __device__ __forceinline__ void store_vec<float>(float* p, const double* o) {
 enum { kCostDense = 0, kCostSqEuclid = 1, kCostL1 = 2 };
 constexpr int kMaxDim = 3;
 
+// a / s, correctly rounded, from y = RN(1/s): one multiply and two Markstein
+// corrections (fused multiply-adds whose residuals a - q*s are exact).  With
+// y within half an ulp of 1/s and the once-corrected quotient within one ulp
+// of a/s, Markstein's theorem makes the second correction the correctly
+// rounded quotient, the value of the IEEE division a / s.  Zero, tiny, huge
+// and non-finite quotients take the division itself.
+__device__ __forceinline__ double exact_quotient(double a, double s, double y) {
+    if (!(a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
+    const double q0 = a * y;
+    const double r0 = fma(-q0, s, a);
+    const double q1 = fma(r0, y, q0);
+    const double r1 = fma(-q1, s, a);
+    return fma(r1, y, q1);
+}
+
+// crcp = 1.0 / cdiv (the IEEE division, once per thread)
 template <int COST>
-__device__ __forceinline__ double point_cost(const double* xi, const double* yj, int dim, double cdiv) {
+__device__ __forceinline__ double point_cost(const double* xi, const double* yj, int dim, double cdiv, double crcp) {
     double c = 0.0;
 #pragma unroll
     for (int d = 0; d < kMaxDim; ++d) {
@@ -432,7 +448,8 @@ __device__ __forceinline__ double point_cost(const double* xi, const double* yj,
             c += (COST == kCostSqEuclid) ? t * t : fabs(t);
         }
     }
-    return cdiv == 1.0 ? c : c / cdiv;  // x / 1 == x: skip the division for unscaled costs
+    // x / 1 == x: no division for unscaled costs
+    return cdiv == 1.0 ? c : exact_quotient(c, cdiv, crcp);
 }
 
 // Everything one block needs to advance one tile of one problem by a PDHG
@@ -506,6 +523,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const double tau = eta / omega;
     const bool gzero = a.gzero;
     const int64_t j0 = a.j0, jstep = a.jstep, ncols = a.ncols;
+    const double crcp = COST != kCostDense ? 1.0 / a.cdiv : 1.0;  // point costs: the reciprocal of the scale
 
     double ui[VEC], uai[VEC];
     bool valid[VEC];
@@ -578,7 +596,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                     unpack<V, T>(B.cv[q], c);
                 } else {
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) c[e] = point_cost<COST>(xi[e], B.yj[q], a.dim, a.cdiv);
+                    for (int e = 0; e < VEC; ++e) c[e] = point_cost<COST>(xi[e], B.yj[q], a.dim, a.cdiv, crcp);
                 }
                 if (!skip_avg) {
                     unpack<V, T>(B.av[q], xa);

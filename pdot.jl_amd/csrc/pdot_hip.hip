@@ -324,13 +324,14 @@ __global__ __launch_bounds__(256) void point_stats_kernel(const double* __restri
                                                           double* __restrict__ part) {
     __shared__ double red[256];
     double mx = 0.0, ss = 0.0;
+    const double crcp = 1.0 / cdiv;
     for (int64_t j = blockIdx.x; j < ncols; j += gridDim.x) {
         double yj[kMaxDim];
         for (int d = 0; d < kMaxDim; ++d) yj[d] = d < dim ? py[(j0 + j) * dim + d] : 0.0;
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
             double xi[kMaxDim];
             for (int d = 0; d < kMaxDim; ++d) xi[d] = d < dim ? px[(int64_t)d * n + i] : 0.0;
-            const double x = point_cost<COST>(xi, yj, dim, cdiv);
+            const double x = point_cost<COST>(xi, yj, dim, cdiv, crcp);
             mx = nanmax(mx, fabs(x));
             ss += x * x;
         }
