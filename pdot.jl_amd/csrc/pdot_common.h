@@ -52,10 +52,10 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 #define PDOT_F64P_WAVES PDOT_F64_WAVES
 #endif
 #ifndef PDOT_F64P_UNROLL
-#define PDOT_F64P_UNROLL PDOT_F64_UNROLL
+#define PDOT_F64P_UNROLL 2  // two columns per batch, no prefetch: measured best with deferred averaging
 #endif
 #ifndef PDOT_F64P_PIPE
-#define PDOT_F64P_PIPE PDOT_F64_PIPE
+#define PDOT_F64P_PIPE 0
 #endif
 #ifndef PDOT_F32P_WAVES
 #define PDOT_F32P_WAVES 4
