@@ -3,12 +3,16 @@ target construction, the closed-form first-evaluation KKT error of the zero
 start used by the warm start's acceptance test (pinned against the oracle's first
 evaluation), the point-cloud problem's dense matrix, and the batch tile-width
 rule's inputs."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
 import pdot_amd as P
 from pdot_amd import instances, learned
+
+REPO_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_normalize_duals_keeps_the_dual_objective_and_balances():
@@ -90,3 +94,23 @@ def test_point_third_evaluation_known_answer_matches_oracle():
     norm, obj = third_evaluation(pts, block=40)
     assert ref["stats"][2]["l2_primal_variable"] == pytest.approx(norm, rel=1e-13)
     assert ref["stats"][2]["primal_objective"] == pytest.approx(obj, rel=1e-13)
+
+
+def test_bench_byte_accounting_matches_the_traffic_profiles():
+    """The bench's algorithmic bytes per step (4 element streams on average with
+    deferred averaging, 3 for point clouds) and the calibrated HBM traffic
+    recorded in profiles/ agree to within the partial sums (under 1%)."""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(REPO_ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    n = 10000
+    assert bench.algorithmic_bytes_per_step(n, n, 8) == 4 * n * n * 8 + 80 * 2 * n
+    assert bench.algorithmic_bytes_per_step(n, n, 8, matrix_streams=3) == 3 * n * n * 8 + 80 * 2 * n
+    for form, elem, streams in (("f64", 8, 4), ("f32", 4, 4), ("f64_points", 8, 3), ("f32_points", 4, 3)):
+        traffic = bench.measured_traffic(n, n, form)
+        alg = bench.algorithmic_bytes_per_step(n, n, elem, matrix_streams=streams)
+        assert 1.0 <= traffic / alg < 1.01, form
+    assert bench.measured_traffic(5000, 5000, "f64") is None
+    bt = bench.measured_batch_traffic()
+    assert 8.8e9 < bt < 9.2e9
