@@ -887,6 +887,9 @@ int pdot_batch_set_states(pdot_batch* b, const pdot_solver_state* in) {
             if (q < 0 || q > 3) return bfail(b, PDOT_E_ARG, "pdot_batch_set_states: slot index out of range");
         if ((s.defer_ok != 0 && s.defer_ok != 1) || (s.avg_pending != 0 && s.avg_pending != 1))
             return bfail(b, PDOT_E_ARG, "pdot_batch_set_states: defer_ok and avg_pending must be 0 or 1");
+        if (s.avg_pending && (s.avg_count == 0 || s.avg_slot == s.cur_slot))
+            return bfail(b, PDOT_E_ARG,
+                         "pdot_batch_set_states: a lagging contribution needs a non-empty average in its own slot");
     }
     memcpy(b->h_st, in, sizeof(pdot_solver_state) * b->count);
     return bwrite_states(b);

@@ -1148,6 +1148,8 @@ int pdot_set_state(pdot_ctx* ctx, const pdot_solver_state* in) {
         return fail(ctx, PDOT_E_ARG, "pdot_set_state: slot index out of range");
     if ((in->defer_ok != 0 && in->defer_ok != 1) || (in->avg_pending != 0 && in->avg_pending != 1))
         return fail(ctx, PDOT_E_ARG, "pdot_set_state: defer_ok and avg_pending must be 0 or 1");
+    if (in->avg_pending && (in->avg_count == 0 || in->avg_slot == in->cur_slot))
+        return fail(ctx, PDOT_E_ARG, "pdot_set_state: a lagging contribution needs a non-empty average in its own slot");
     return write_state(ctx, in);
 }
 

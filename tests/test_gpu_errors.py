@@ -71,6 +71,10 @@ def test_state_and_defer_arguments_are_validated():
         setattr(bad, field, value)
         with pytest.raises(_lib.PdotError, match=msg):
             dev.set_state(bad)
+    bad = _lib.SolverState.from_buffer_copy(st)
+    bad.avg_slot = bad.cur_slot
+    with pytest.raises(_lib.PdotError, match="lagging contribution"):
+        dev.set_state(bad)
     with pytest.raises(_lib.PdotError, match="0 or 1"):
         dev._check(dev.lib.pdot_set_defer(dev.h, 2), "pdot_set_defer")
     dev.set_state(st)
@@ -84,9 +88,10 @@ def test_state_and_defer_arguments_are_validated():
     b.set_start(_lib.POLICY_ADAPTIVE, 2.0, 1.0, 1.0, 1.0)
     states = b.run_steps([2, 3])
     assert [s.avg_pending for s in states] == [1, 0]
-    for field, value, msg in (("avg_slot", 9, "slot index"), ("avg_pending", 3, "0 or 1")):
+    for k, field, value, msg in ((1, "avg_slot", 9, "slot index"), (1, "avg_pending", 3, "0 or 1"),
+                                 (0, "avg_count", 0, "lagging contribution")):
         bad = (_lib.SolverState * 2).from_buffer_copy(states)
-        setattr(bad[1], field, value)
+        setattr(bad[k], field, value)
         with pytest.raises(_lib.PdotError, match=msg):
             b._check(b.lib.pdot_batch_set_states(b.h, bad), "pdot_batch_set_states")
     with pytest.raises(_lib.PdotError, match="0 or 1"):
