@@ -427,9 +427,10 @@ constexpr int kMaxDim = 3;
 // y within half an ulp of 1/s and the once-corrected quotient within one ulp
 // of a/s, Markstein's theorem makes the second correction the correctly
 // rounded quotient, the value of the IEEE division a / s.  Zero, tiny, huge
-// and non-finite quotients take the division itself.
+// and non-finite quotients, and scales whose reciprocal could lose precision,
+// take the division itself.
 __device__ __forceinline__ double exact_quotient(double a, double s, double y) {
-    if (!(a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
+    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
     const double q0 = a * y;
     const double r0 = fma(-q0, s, a);
     const double q1 = fma(r0, y, q0);

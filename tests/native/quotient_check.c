@@ -8,7 +8,7 @@
 #include <stdint.h>
 
 static double quotient(double a, double s, double y) {
-    if (!(a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
+    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
     const double q0 = a * y;
     const double r0 = fma(-q0, s, a);
     const double q1 = fma(r0, y, q0);

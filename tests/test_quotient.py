@@ -33,7 +33,8 @@ def _lib(tmp_path):
 def test_quotient_sequence_is_the_correctly_rounded_division(tmp_path):
     lib = _lib(tmp_path)
     rng = np.random.default_rng(7)
-    scales = np.concatenate([[1.0, 3.0, 0.1, 7.0 / 3.0, 2.0 ** 0.5, np.nextafter(2.0, 0), np.nextafter(1.0, 2)],
+    scales = np.concatenate([[1.0, 3.0, 0.1, 7.0 / 3.0, 2.0 ** 0.5, np.nextafter(2.0, 0), np.nextafter(1.0, 2),
+                              1e305, 3e-305, 2.0 ** 1000, 2.0 ** -1000],
                              10.0 ** rng.uniform(-6, 6, 25), rng.uniform(1, 2, 25)])
     for k, s in enumerate(scales):
         assert lib.quotient_mismatches_random(float(s), 200_000, 1000 + k) == 0, s
