@@ -234,3 +234,41 @@ def test_batch_lagging_average_reads_equal_eager_updates(storage):
     both(lambda b: b.discard([1, 1, 1]))
     check()
     both(lambda b: b.close())
+
+
+def _policy_params(kind):
+    p = P.default_parameters(1e-4, iteration_limit=6000)
+    if kind == "constant_step":
+        p.step_size_policy_params = P.ConstantStepsizeParams()
+    elif kind == "fixed_frequency":
+        p.restart_params = P.construct_restart_parameters(P.FIXED_FREQUENCY, P.KKT_GREEDY, 256, 0.9, 0.1, 0.9, 0.5)
+    elif kind == "no_restarts":
+        p.restart_params = P.construct_restart_parameters(P.NO_RESTARTS, P.KKT_GREEDY, 1000, 0.36, 0.1, 0.9, 0.5)
+    elif kind == "no_restart_to_current":
+        p.restart_params = P.construct_restart_parameters(P.ADAPTIVE_KKT, P.NO_RESTART_TO_CURRENT, 1000, 0.36, 0.1,
+                                                          0.9, 0.5)
+    return p
+
+
+@pytest.mark.parametrize("kind", ["constant_step", "fixed_frequency", "no_restarts", "no_restart_to_current"])
+def test_deferred_policy_variants_equal_eager_bitwise(kind):
+    """The step-size and restart policies change when the average is reset,
+    restarted to or read; the deferred and eager runs must still agree bit
+    for bit."""
+    prob = instances.uniform_sqeuclid(90, 70, seed=31)
+    outs = []
+    for defer in (True, False):
+        dev = _device(prob, _lib.STORAGE_F64, defer, False)
+        outs.append(P.optimize(_policy_params(kind), prob, solver=dev))
+        dev.close()
+    a, b = outs
+    assert a.termination_reason == b.termination_reason
+    assert a.iteration_count == b.iteration_count > 100
+    assert len(a.iteration_stats) == len(b.iteration_stats)
+    for sa, sb in zip(a.iteration_stats, b.iteration_stats):
+        assert sa.restart_used == sb.restart_used
+        assert sa.step_size == sb.step_size and sa.primal_weight == sb.primal_weight
+        for ca, cb in zip(sa.convergence_information, sb.convergence_information):
+            assert ca.primal_objective == cb.primal_objective and ca.l2_dual_residual == cb.l2_dual_residual
+    _same_arrays((a.primal_solution, a.dual_source_solution, a.dual_target_solution),
+                 (b.primal_solution, b.dual_source_solution, b.dual_target_solution))
