@@ -250,16 +250,21 @@ def _policy_params(kind):
     return p
 
 
-@pytest.mark.parametrize("kind", ["constant_step", "fixed_frequency", "no_restarts", "no_restart_to_current"])
+@pytest.mark.parametrize("kind", ["constant_step", "fixed_frequency", "no_restarts", "no_restart_to_current",
+                                  "warm_start"])
 def test_deferred_policy_variants_equal_eager_bitwise(kind):
     """The step-size and restart policies change when the average is reset,
     restarted to or read; the deferred and eager runs must still agree bit
-    for bit."""
+    for bit (also from a warm start, where G starts from the given duals)."""
     prob = instances.uniform_sqeuclid(90, 70, seed=31)
+    ws = None
+    if kind == "warm_start":
+        rng = np.random.default_rng(3)
+        ws = (0.01 * rng.standard_normal(90), 0.01 * rng.standard_normal(70))
     outs = []
     for defer in (True, False):
         dev = _device(prob, _lib.STORAGE_F64, defer, False)
-        outs.append(P.optimize(_policy_params(kind), prob, solver=dev))
+        outs.append(P.optimize(_policy_params(kind), prob, solver=dev, warm_start=ws))
         dev.close()
     a, b = outs
     assert a.termination_reason == b.termination_reason
