@@ -79,6 +79,41 @@ class OracleStats(ctypes.Structure):
     _fields_ = STAT_FIELDS
 
 
+RESTART_FIELDS = [
+    ("iteration", ctypes.c_int32),
+    ("restart_used", ctypes.c_int32),
+    ("artificial", ctypes.c_int32),
+    ("adaptive_checked", ctypes.c_int32),
+    ("reset_to_average", ctypes.c_int32),
+    ("pad", ctypes.c_int32),
+    ("avg_count", ctypes.c_int64),
+    ("iterations_completed", ctypes.c_int64),
+    ("step_size", ctypes.c_double),
+    ("primal_weight_before", ctypes.c_double),
+    ("primal_weight_after", ctypes.c_double),
+    ("kkt_current", ctypes.c_double),
+    ("kkt_average", ctypes.c_double),
+    ("kkt_last", ctypes.c_double),
+    ("ratio", ctypes.c_double),
+    ("last_trial_before", ctypes.c_double),
+    ("last_trial_after", ctypes.c_double),
+    ("last_x_norm", ctypes.c_double),
+    ("last_y_norm", ctypes.c_double),
+    ("primal_distance", ctypes.c_double),
+    ("dual_distance", ctypes.c_double),
+    ("pieces", (ctypes.c_double * 6) * 3),
+]
+
+# order of a KKT-pieces row in oracle_restart_rec.pieces
+PIECE_FIELDS = ("l2_primal_residual", "l2_dual_residual", "primal_objective", "dual_objective", "x_norm", "y_norm")
+
+
+class OracleRestartRec(ctypes.Structure):
+    """oracle_restart_rec: what run_restart_scheme read and decided at one
+    evaluation."""
+    _fields_ = RESTART_FIELDS
+
+
 _dp = ctypes.POINTER(ctypes.c_double)
 _lib = None
 
@@ -99,6 +134,21 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(OracleStats), ctypes.c_int32,
                                       ctypes.POINTER(ctypes.c_int32), _dp, ctypes.c_int64,
                                       ctypes.POINTER(ctypes.c_int64)]
+        L.oracle_optimize_ex.restype = ctypes.c_int
+        L.oracle_optimize_ex.argtypes = L.oracle_optimize.argtypes + [ctypes.POINTER(OracleRestartRec),
+                                                                       ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]
+        L.oracle_weighted_kkt.restype = ctypes.c_double
+        L.oracle_weighted_kkt.argtypes = [ctypes.c_double] * 5
+        L.oracle_should_do_adaptive_restart_kkt.restype = ctypes.c_int
+        L.oracle_should_do_adaptive_restart_kkt.argtypes = [ctypes.c_double] * 4 + [_dp]
+        L.oracle_restart_decision.restype = ctypes.c_int
+        L.oracle_restart_decision.argtypes = [ctypes.POINTER(OracleParams), ctypes.c_int64, ctypes.c_int64,
+                                              ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
+                                              ctypes.POINTER(OracleRestartRec)]
+        L.oracle_restart_distances.restype = None
+        L.oracle_restart_distances.argtypes = [ctypes.c_double] * 5 + [_dp, _dp]
+        L.oracle_new_primal_weight.restype = ctypes.c_double
+        L.oracle_new_primal_weight.argtypes = [ctypes.c_double] * 4
         L.oracle_fused_pass.restype = None
         L.oracle_fused_pass.argtypes = [ctypes.c_int64, ctypes.c_int64, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
                                         ctypes.c_int32, ctypes.c_double, ctypes.c_double, _dp, _dp, _dp, _dp, _dp,
@@ -149,9 +199,11 @@ def params_from(p) -> OracleParams:
     return o
 
 
-def optimize(params, problem, mode: int = FUSED, warm_start=None, stats_cap: int = 200000, trace_cap: int = 0):
+def optimize(params, problem, mode: int = FUSED, warm_start=None, stats_cap: int = 200000, trace_cap: int = 0,
+             restart_cap: int = 0):
     """Run the oracle solve; returns a dict with X, u, v, termination_reason,
-    iteration_count, stats (list of dicts) and eta_trace."""
+    iteration_count, stats (list of dicts), eta_trace and, with restart_cap
+    > 0, restart_trace (list of dicts, one per non-terminating evaluation)."""
     L = lib()
     C = np.asfortranarray(problem.cost_matrix, dtype=np.float64)
     a = np.ascontiguousarray(problem.source_distribution, dtype=np.float64)
@@ -173,15 +225,60 @@ def optimize(params, problem, mode: int = FUSED, warm_start=None, stats_cap: int
         if len(warm_start) > 2 and warm_start[2] is not None:
             X0 = np.asfortranarray(warm_start[2], dtype=np.float64)
     op = params_from(params)
-    rc = L.oracle_optimize(ctypes.byref(op), n, m, _p(C), _p(a), _p(b), int(mode), _p(u0), _p(v0), _p(X0), _p(X), _p(u),
-                           _p(v), ctypes.byref(reason), ctypes.byref(iters), stats, stats_cap, ctypes.byref(nst),
-                           _p(trace), trace_cap, ctypes.byref(tlen))
+    rtrace = (OracleRestartRec * max(1, restart_cap))()
+    nrt = ctypes.c_int32()
+    rc = L.oracle_optimize_ex(ctypes.byref(op), n, m, _p(C), _p(a), _p(b), int(mode), _p(u0), _p(v0), _p(X0), _p(X),
+                              _p(u), _p(v), ctypes.byref(reason), ctypes.byref(iters), stats, stats_cap,
+                              ctypes.byref(nst), _p(trace), trace_cap, ctypes.byref(tlen), rtrace, restart_cap,
+                              ctypes.byref(nrt))
     if rc != 0:
         raise RuntimeError(f"oracle_optimize failed with code {rc}")
     rec = [{k: getattr(stats[i], k) for k, _ in STAT_FIELDS} for i in range(min(nst.value, stats_cap))]
+    rrec = [restart_rec_dict(rtrace[i]) for i in range(min(nrt.value, restart_cap))]
     return {"X": X, "u": u, "v": v, "termination_reason": reason.value, "iteration_count": iters.value,
             "stats": rec, "n_stats": nst.value, "eta_trace": trace[:min(tlen.value, trace_cap)],
-            "take_step_calls": tlen.value}
+            "take_step_calls": tlen.value, "restart_trace": rrec, "n_restart_trace": nrt.value}
+
+
+def restart_rec_dict(r) -> dict:
+    d = {k: getattr(r, k) for k, _ in RESTART_FIELDS if k not in ("pad", "pieces")}
+    for q, name in enumerate(("current", "average", "last")):
+        d[name] = {f: r.pieces[q][i] for i, f in enumerate(PIECE_FIELDS)}
+    return d
+
+
+# ---- the scalar restart functions of the oracle (saddle_point_gpu.jl:299-330, 492-780)
+
+def weighted_kkt(omega, p2, d2, pobj, dobj) -> float:
+    return float(lib().oracle_weighted_kkt(omega, p2, d2, pobj, dobj))
+
+
+def should_do_adaptive_restart_kkt(candidate_kkt, last_kkt, sufficient, necessary, last_trial):
+    """Returns (do_restart, last_trial after the trial)."""
+    lt = ctypes.c_double(last_trial)
+    r = lib().oracle_should_do_adaptive_restart_kkt(candidate_kkt, last_kkt, sufficient, necessary, ctypes.byref(lt))
+    return bool(r), lt.value
+
+
+def restart_decision(params, avg_count, iterations_completed, kc, ka, kl, last_trial):
+    """Returns (RestartChoice value, last_trial after, record dict)."""
+    op = params_from(params)
+    lt = ctypes.c_double(last_trial)
+    rec = OracleRestartRec()
+    choice = lib().oracle_restart_decision(ctypes.byref(op), int(avg_count), int(iterations_completed), kc, ka, kl,
+                                           ctypes.byref(lt), ctypes.byref(rec))
+    return int(choice), lt.value, restart_rec_dict(rec)
+
+
+def restart_distances(last_x_norm, last_y_norm, primal_norm_params, dual_norm_params, primal_weight):
+    pd, dd = ctypes.c_double(), ctypes.c_double()
+    lib().oracle_restart_distances(last_x_norm, last_y_norm, primal_norm_params, dual_norm_params, primal_weight,
+                                   ctypes.byref(pd), ctypes.byref(dd))
+    return pd.value, dd.value
+
+
+def new_primal_weight(primal_distance, dual_distance, primal_weight, smoothing) -> float:
+    return float(lib().oracle_new_primal_weight(primal_distance, dual_distance, primal_weight, smoothing))
 
 
 def fused_pass(C, a, b, X, u, v, r, c, gzero, eta, omega, Xavg, uavg, vavg, ravg, cavg, S):

@@ -493,10 +493,121 @@ static void convergence_info(const ot_cache_t* oc, double eps_ratio, const matst
     s->candidate_type = POINT_TYPE_AVERAGE_ITERATE; /* solver_gpu.jl:551, 571 */
 }
 
-/* compute_weight_kkt_residual saddle_point_gpu.jl:299-330 */
-static double weighted_kkt(double omega, double p2, double d2, double pobj, double dobj) {
+/* compute_weight_kkt_residual saddle_point_gpu.jl:299-330, from the point's
+ * L2 primal residual p2, L2 dual residual d2 and objectives. */
+double oracle_weighted_kkt(double omega, double p2, double d2, double pobj, double dobj) {
     double g = fabs(pobj - dobj);
     return sqrt(omega * (p2 * p2) + 1 / omega * (d2 * d2) + g * g);
+}
+
+/* ------------------------------------------------- restart decisions */
+
+/* One record per evaluation that reaches run_restart_scheme (every
+ * non-terminating evaluation): what the restart scheme read and decided. */
+typedef struct {
+    int32_t iteration;        /* the evaluation iteration */
+    int32_t restart_used;     /* RestartChoice */
+    int32_t artificial;       /* restart_length >= threshold * iterations_completed (:561-564) */
+    int32_t adaptive_checked; /* should_do_adaptive_restart_kkt ran */
+    int32_t reset_to_average; /* should_reset_to_average (:477-487) */
+    int32_t pad;
+    int64_t avg_count;        /* restart_length (:558) */
+    int64_t iterations_completed;
+    double step_size, primal_weight_before, primal_weight_after;
+    double kkt_current, kkt_average, kkt_last; /* weighted KKT at primal_weight_before (NaN: not formed) */
+    double ratio;                              /* candidate / last (NaN unless checked) */
+    double last_trial_before, last_trial_after;
+    double last_x_norm, last_y_norm;           /* norms of the previous restart point (restarts only) */
+    double primal_distance, dual_distance;     /* update_last_restart_info! (restarts only) */
+    /* (p2, d2, pobj, dobj, ||X||, ||[u;v]||) of the current point, the average and the last restart point */
+    double pieces[3][6];
+} oracle_restart_rec;
+
+/* should_do_adaptive_restart_kkt saddle_point_gpu.jl:492-533, given the
+ * weighted KKT of the candidate and of the last restart point, both at the
+ * current primal weight.  The trial's ratio is stored in *last_trial whatever
+ * the outcome (:530). */
+int oracle_should_do_adaptive_restart_kkt(double candidate_kkt, double last_kkt, double sufficient,
+                                          double necessary, double* last_trial) {
+    int do_restart = 0;
+    double ratio = candidate_kkt / last_kkt;
+    if (ratio < necessary) {
+        if (ratio < sufficient) do_restart = 1;
+        else if (ratio > *last_trial) do_restart = 1;
+    }
+    *last_trial = ratio;
+    return do_restart;
+}
+
+/* The decision of run_restart_scheme (saddle_point_gpu.jl:539-633) from the
+ * weighted KKT residuals at the current primal weight: kc of the current
+ * point, ka of the average, kl of the last restart point (kc and ka are not
+ * read under NO_RESTARTS, kl only where the adaptive test runs).  Returns the
+ * RestartChoice; *last_trial is the CuRestartInfo field; rec (may be NULL)
+ * receives the flags and the ratio. */
+int oracle_restart_decision(const oracle_params* p, int64_t avg_count, int64_t iterations_completed, double kc,
+                            double ka, double kl, double* last_trial, oracle_restart_rec* rec) {
+    if (rec) {
+        rec->artificial = rec->adaptive_checked = rec->reset_to_average = 0;
+        rec->ratio = NAN;
+        rec->last_trial_before = *last_trial;
+    }
+    if (!(avg_count > 0)) { /* :558-562 */
+        if (rec) rec->last_trial_after = *last_trial;
+        return RESTART_CHOICE_NO_RESTART;
+    }
+    int do_restart = 0, reset_to_average = 0;
+    double cand = NAN;
+    if ((double)avg_count >= p->artificial_restart_threshold * (double)iterations_completed) { /* :568-571 */
+        do_restart = 1;
+        if (rec) rec->artificial = 1;
+    }
+    if (p->restart_scheme != NO_RESTARTS) { /* :573-615 */
+        reset_to_average = (p->restart_to_current_metric == KKT_GREEDY) ? (kc >= ka) : 1;
+        cand = reset_to_average ? ka : kc;
+    }
+    if (!do_restart) { /* :617-633 */
+        if (p->restart_scheme == ADAPTIVE_KKT) {
+            do_restart = oracle_should_do_adaptive_restart_kkt(cand, kl, p->sufficient_reduction_for_restart,
+                                                               p->necessary_reduction_for_restart, last_trial);
+            if (rec) {
+                rec->adaptive_checked = 1;
+                rec->ratio = *last_trial;
+            }
+        } else if (p->restart_scheme == FIXED_FREQUENCY && p->restart_frequency_if_fixed <= avg_count) {
+            do_restart = 1;
+        }
+    }
+    if (rec) {
+        rec->reset_to_average = reset_to_average;
+        rec->last_trial_after = *last_trial;
+    }
+    if (!do_restart) return RESTART_CHOICE_NO_RESTART;
+    return reset_to_average ? RESTART_CHOICE_RESTART_TO_AVERAGE : RESTART_CHOICE_WEIGHTED_AVERAGE_RESET;
+}
+
+/* update_last_restart_info! saddle_point_gpu.jl:753-767 after the reset of
+ * the average (:667): the distances moved are the weighted norms
+ * (primal_weighted_norm / dual_weighted_norm, :36-50) of 0 minus the previous
+ * restart point, whose norms are last_x_norm = ||X|| and last_y_norm =
+ * ||[u; v]||. */
+void oracle_restart_distances(double last_x_norm, double last_y_norm, double primal_norm_params,
+                              double dual_norm_params, double primal_weight, double* primal_distance,
+                              double* dual_distance) {
+    *primal_distance = sqrt(primal_norm_params) * last_x_norm / sqrt(primal_weight);
+    *dual_distance = sqrt(dual_norm_params) * last_y_norm * sqrt(primal_weight);
+}
+
+/* compute_new_primal_weight saddle_point_gpu.jl:701-728 */
+double oracle_new_primal_weight(double primal_distance, double dual_distance, double primal_weight,
+                                double smoothing) {
+    const double eps = 2.220446049250313e-16; /* eps(Float64) */
+    if (primal_distance > eps && dual_distance > eps) {
+        double est = dual_distance / primal_distance;
+        double lw = smoothing * log(est) + (1 - smoothing) * log(primal_weight);
+        return exp(lw);
+    }
+    return primal_weight;
 }
 
 /* optimality_criteria_met termination.jl:106-134 */
@@ -552,11 +663,15 @@ static grad_t point_grad(const point_t* pt) {
     return g;
 }
 
-int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* C, const double* a,
-                    const double* b, int32_t mode, const double* u0, const double* v0, const double* X0, double* X_out,
-                    double* u_out, double* v_out, int32_t* termination_reason, int32_t* iteration_count,
-                    oracle_stats* stats, int32_t stats_cap, int32_t* n_stats, double* eta_trace,
-                    int64_t trace_cap, int64_t* trace_len) {
+/* optimize (solver_gpu.jl:413-720).  stats receives the recorded
+ * IterationStats, eta_trace the step size after every take_step! call and
+ * rtrace (may be NULL) one oracle_restart_rec per non-terminating evaluation. */
+int oracle_optimize_ex(const oracle_params* p, int64_t n, int64_t m, const double* C, const double* a,
+                       const double* b, int32_t mode, const double* u0, const double* v0, const double* X0,
+                       double* X_out, double* u_out, double* v_out, int32_t* termination_reason,
+                       int32_t* iteration_count, oracle_stats* stats, int32_t stats_cap, int32_t* n_stats,
+                       double* eta_trace, int64_t trace_cap, int64_t* trace_len, oracle_restart_rec* rtrace,
+                       int32_t rtrace_cap, int32_t* n_rtrace) {
     const int literal = (mode == ORACLE_LITERAL);
     if (n <= 0 || m <= 0 || p->termination_evaluation_frequency <= 0) return -4;
     /* solver_gpu.jl:420-422 */
@@ -641,6 +756,7 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
     int64_t total_number_iterations = 0;
 
     int32_t nst = 0;
+    int32_t nrt = 0;
     int64_t ntrace = 0;
     double start_time = wall_time();
     double time_basic = 0.0;
@@ -734,75 +850,60 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
             }
 
             /* run_restart_scheme saddle_point_gpu.jl:539-696 */
-            int restart_used;
-            if (!(avg.primal_count > 0 && avg.dual_source_count > 0 && avg.dual_target_count > 0)) {
-                restart_used = RESTART_CHOICE_NO_RESTART;
-            } else {
-                int64_t iterations_completed = iteration - 1;
-                int64_t restart_length = avg.primal_count;
-                int artificial = 0, do_restart = 0, reset_to_average = 0;
-                if ((double)restart_length >= p->artificial_restart_threshold * (double)iterations_completed) {
-                    do_restart = 1;
-                    artificial = 1;
-                }
-                (void)artificial;
-                double cand_kkt = NAN, cand_p2 = 0, cand_d2 = 0, cand_pobj = 0, cand_dobj = 0;
-                double cur_x_sq = 0, cur_y_sq = 0;
-                int have_cand = 0;
-                if (p->restart_scheme == NO_RESTARTS) {
-                    reset_to_average = 0;
+            int restart_used = RESTART_CHOICE_NO_RESTART;
+            oracle_restart_rec rr;
+            memset(&rr, 0, sizeof(rr));
+            rr.iteration = (int32_t)iteration;
+            rr.avg_count = avg.primal_count;
+            rr.iterations_completed = iteration - 1;
+            rr.step_size = step_size;
+            rr.primal_weight_before = primal_weight;
+            rr.kkt_current = rr.kkt_average = rr.kkt_last = rr.ratio = NAN;
+            rr.last_x_norm = rr.last_y_norm = rr.primal_distance = rr.dual_distance = NAN;
+            for (int q = 0; q < 3; ++q)
+                for (int f = 0; f < 6; ++f) rr.pieces[q][f] = NAN;
+            if (avg.primal_count > 0 && avg.dual_source_count > 0 && avg.dual_target_count > 0) {
+                /* the weighted KKT residuals the scheme reads (:575-612, 505-515) */
+                matstats_t msc, msa;
+                vecstats_t vsc, vsa;
+                grad_t gc = point_grad(&cur);
+                point_matstats(&P, cur.X, &gc, scratch, &msc);
+                point_vecstats(&P, cur.u, cur.v, cur.r, cur.c, &vsc);
+                grad_t ga = {avg.G, 0, avg.u, avg.v};
+                point_matstats(&P, avg.X, &ga, scratch, &msa);
+                point_vecstats(&P, avg.u, avg.v, avg.r, avg.c, &vsa);
+                double pc[6] = {sqrt(vsc.p_sq), sqrt(msc.rc_sq), msc.pobj, vsc.dobj, sqrt(msc.x_sq), sqrt(vsc.y_sq)};
+                double pa[6] = {sqrt(vsa.p_sq), sqrt(msa.rc_sq), msa.pobj, vsa.dobj, sqrt(msa.x_sq), sqrt(vsa.y_sq)};
+                double pl[6];
+                if (literal) {
+                    matstats_t msl;
+                    vecstats_t vsl;
+                    grad_t gl = {last.G, 0, last.u, last.v};
+                    point_matstats(&P, last.X, &gl, scratch, &msl);
+                    point_vecstats(&P, last.u, last.v, last.r, last.c, &vsl);
+                    double t[6] = {sqrt(vsl.p_sq), sqrt(msl.rc_sq), msl.pobj, vsl.dobj, sqrt(msl.x_sq), sqrt(vsl.y_sq)};
+                    memcpy(pl, t, sizeof(pl));
                 } else {
-                    matstats_t msc, msa;
-                    vecstats_t vsc, vsa;
-                    grad_t gc = point_grad(&cur);
-                    point_matstats(&P, cur.X, &gc, scratch, &msc);
-                    point_vecstats(&P, cur.u, cur.v, cur.r, cur.c, &vsc);
-                    grad_t ga = {avg.G, 0, avg.u, avg.v};
-                    point_matstats(&P, avg.X, &ga, scratch, &msa);
-                    point_vecstats(&P, avg.u, avg.v, avg.r, avg.c, &vsa);
-                    double kc = weighted_kkt(primal_weight, sqrt(vsc.p_sq), sqrt(msc.rc_sq), msc.pobj, vsc.dobj);
-                    double ka = weighted_kkt(primal_weight, sqrt(vsa.p_sq), sqrt(msa.rc_sq), msa.pobj, vsa.dobj);
-                    /* should_reset_to_average :477-487 */
-                    reset_to_average = (p->restart_to_current_metric == KKT_GREEDY) ? (kc >= ka) : 1;
-                    have_cand = 1;
-                    if (reset_to_average) {
-                        cand_kkt = ka; cand_p2 = sqrt(vsa.p_sq); cand_d2 = sqrt(msa.rc_sq);
-                        cand_pobj = msa.pobj; cand_dobj = vsa.dobj;
-                        cur_x_sq = msa.x_sq; cur_y_sq = vsa.y_sq;
-                    } else {
-                        cand_kkt = kc; cand_p2 = sqrt(vsc.p_sq); cand_d2 = sqrt(msc.rc_sq);
-                        cand_pobj = msc.pobj; cand_dobj = vsc.dobj;
-                        cur_x_sq = msc.x_sq; cur_y_sq = vsc.y_sq;
-                    }
+                    double t[6] = {last.p2, last.d2, last.pobj, last.dobj, last.x_norm, last.y_norm};
+                    memcpy(pl, t, sizeof(pl));
                 }
-                if (!do_restart) {
-                    if (p->restart_scheme == ADAPTIVE_KKT) {
-                        /* should_do_adaptive_restart_kkt :492-533 */
-                        double kl;
-                        if (literal) {
-                            matstats_t msl;
-                            vecstats_t vsl;
-                            grad_t gl = {last.G, 0, last.u, last.v};
-                            point_matstats(&P, last.X, &gl, scratch, &msl);
-                            point_vecstats(&P, last.u, last.v, last.r, last.c, &vsl);
-                            kl = weighted_kkt(primal_weight, sqrt(vsl.p_sq), sqrt(msl.rc_sq), msl.pobj, vsl.dobj);
-                        } else {
-                            kl = weighted_kkt(primal_weight, last.p2, last.d2, last.pobj, last.dobj);
-                        }
-                        double ratio = cand_kkt / kl;
-                        if (ratio < p->necessary_reduction_for_restart) {
-                            if (ratio < p->sufficient_reduction_for_restart) do_restart = 1;
-                            else if (ratio > last.kkt_reduction_ratio_last_trial) do_restart = 1;
-                        }
-                        last.kkt_reduction_ratio_last_trial = ratio;
-                    } else if (p->restart_scheme == FIXED_FREQUENCY && p->restart_frequency_if_fixed <= restart_length) {
-                        do_restart = 1;
-                    }
+                memcpy(rr.pieces[0], pc, sizeof(pc));
+                memcpy(rr.pieces[1], pa, sizeof(pa));
+                memcpy(rr.pieces[2], pl, sizeof(pl));
+                if (p->restart_scheme != NO_RESTARTS) {
+                    rr.kkt_current = oracle_weighted_kkt(primal_weight, pc[0], pc[1], pc[2], pc[3]);
+                    rr.kkt_average = oracle_weighted_kkt(primal_weight, pa[0], pa[1], pa[2], pa[3]);
                 }
-                if (!do_restart) {
-                    restart_used = RESTART_CHOICE_NO_RESTART;
-                } else {
-                    if (reset_to_average) { /* :646-651 */
+                rr.kkt_last = oracle_weighted_kkt(primal_weight, pl[0], pl[1], pl[2], pl[3]);
+                restart_used = oracle_restart_decision(p, avg.primal_count, iteration - 1, rr.kkt_current,
+                                                       rr.kkt_average, rr.kkt_last,
+                                                       &last.kkt_reduction_ratio_last_trial, &rr);
+                if (restart_used != RESTART_CHOICE_NO_RESTART) {
+                    const int to_average = (restart_used == RESTART_CHOICE_RESTART_TO_AVERAGE);
+                    /* the point the scheme restarts from: under NO_RESTARTS (no
+                     * candidate) the current point (:573-576, 654-657) */
+                    const double* pn = to_average ? pa : pc;
+                    if (to_average) { /* :646-651 */
                         memcpy(cur.X, avg.X, sizeof(double) * nm);
                         memcpy(cur.u, avg.u, sizeof(double) * n);
                         memcpy(cur.v, avg.v, sizeof(double) * m);
@@ -810,16 +911,6 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
                         memcpy(cur.c, avg.c, sizeof(double) * m);
                         if (literal) memcpy(cur.G, avg.G, sizeof(double) * nm);
                         cur.gzero = 0;
-                    }
-                    if (!have_cand) { /* NO_RESTARTS scheme: artificial restart to current */
-                        matstats_t msc;
-                        vecstats_t vsc;
-                        grad_t gc = point_grad(&cur);
-                        point_matstats(&P, cur.X, &gc, scratch, &msc);
-                        point_vecstats(&P, cur.u, cur.v, cur.r, cur.c, &vsc);
-                        cand_p2 = sqrt(vsc.p_sq); cand_d2 = sqrt(msc.rc_sq);
-                        cand_pobj = msc.pobj; cand_dobj = vsc.dobj;
-                        cur_x_sq = msc.x_sq; cur_y_sq = vsc.y_sq;
                     }
                     reset_average(&P, &avg); /* :667 */
                     /* update_last_restart_info! :733-780; the average was just
@@ -845,8 +936,12 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
                         last_x_norm = last.x_norm;
                         last_y_norm = last.y_norm;
                     }
-                    last.primal_distance = sqrt(primal_norm_params) * last_x_norm / sqrt(primal_weight);
-                    last.dual_distance = sqrt(dual_norm_params) * last_y_norm * sqrt(primal_weight);
+                    oracle_restart_distances(last_x_norm, last_y_norm, primal_norm_params, dual_norm_params,
+                                             primal_weight, &last.primal_distance, &last.dual_distance);
+                    rr.last_x_norm = last_x_norm;
+                    rr.last_y_norm = last_y_norm;
+                    rr.primal_distance = last.primal_distance;
+                    rr.dual_distance = last.dual_distance;
                     if (literal) {
                         memcpy(last.X, cur.X, sizeof(double) * nm);
                         memcpy(last.u, cur.u, sizeof(double) * n);
@@ -855,29 +950,28 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
                         memcpy(last.c, cur.c, sizeof(double) * m);
                         memcpy(last.G, cur.G, sizeof(double) * nm);
                     } else {
-                        last.x_norm = sqrt(cur_x_sq);
-                        last.y_norm = sqrt(cur_y_sq);
-                        last.p2 = cand_p2;
-                        last.d2 = cand_d2;
-                        last.pobj = cand_pobj;
-                        last.dobj = cand_dobj;
+                        last.p2 = pn[0];
+                        last.d2 = pn[1];
+                        last.pobj = pn[2];
+                        last.dobj = pn[3];
+                        last.x_norm = pn[4];
+                        last.y_norm = pn[5];
                     }
-                    last.last_restart_length = restart_length;
-                    restart_used = reset_to_average ? RESTART_CHOICE_RESTART_TO_AVERAGE
-                                                    : RESTART_CHOICE_WEIGHTED_AVERAGE_RESET;
+                    last.last_restart_length = rr.avg_count;
                 }
+            } else {
+                rr.last_trial_before = rr.last_trial_after = last.kkt_reduction_ratio_last_trial;
             }
             if (rec_idx >= 0) stats[rec_idx].restart_used = restart_used;
             if (restart_used != RESTART_CHOICE_NO_RESTART) {
                 /* compute_new_primal_weight saddle_point_gpu.jl:701-728 */
-                double pd = last.primal_distance, dd = last.dual_distance;
-                const double eps = 2.220446049250313e-16;
-                if (pd > eps && dd > eps) {
-                    double est = dd / pd;
-                    double lw = smoothing * log(est) + (1 - smoothing) * log(primal_weight);
-                    primal_weight = exp(lw);
-                }
+                primal_weight = oracle_new_primal_weight(last.primal_distance, last.dual_distance, primal_weight,
+                                                         smoothing);
             }
+            rr.restart_used = restart_used;
+            rr.primal_weight_after = primal_weight;
+            if (rtrace && nrt < rtrace_cap) rtrace[nrt] = rr;
+            nrt++;
         }
 
         double t_basic0 = wall_time();
@@ -973,10 +1067,21 @@ int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* 
     }
     *n_stats = nst;
     if (trace_len) *trace_len = ntrace;
+    if (n_rtrace) *n_rtrace = nrt;
     free(mats);
     free(vecs);
     free(scratch);
     return rc;
+}
+
+int oracle_optimize(const oracle_params* p, int64_t n, int64_t m, const double* C, const double* a,
+                    const double* b, int32_t mode, const double* u0, const double* v0, const double* X0, double* X_out,
+                    double* u_out, double* v_out, int32_t* termination_reason, int32_t* iteration_count,
+                    oracle_stats* stats, int32_t stats_cap, int32_t* n_stats, double* eta_trace,
+                    int64_t trace_cap, int64_t* trace_len) {
+    return oracle_optimize_ex(p, n, m, C, a, b, mode, u0, v0, X0, X_out, u_out, v_out, termination_reason,
+                              iteration_count, stats, stats_cap, n_stats, eta_trace, trace_cap, trace_len, NULL, 0,
+                              NULL);
 }
 
 /* --------------------------------------------- single-pass primitives */

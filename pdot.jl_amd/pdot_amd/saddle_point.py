@@ -132,8 +132,25 @@ def should_do_adaptive_restart_kkt(candidate_kkt: float, restart_params: Restart
 
 @dataclass
 class RestartDecision:
+    """The outcome of run_restart_scheme and what it read: the weighted KKT
+    residuals (NaN where not formed), the ratio of the adaptive test (NaN
+    unless it ran), kkt_reduction_ratio_last_trial before and after, and on a
+    restart the norms of the previous restart point and the distances moved."""
     choice: RestartChoice
     to_average: bool = False
+    artificial: bool = False
+    adaptive_checked: bool = False
+    reset_to_average: bool = False
+    kkt_current: float = math.nan
+    kkt_average: float = math.nan
+    kkt_last: float = math.nan
+    ratio: float = math.nan
+    last_trial_before: float = math.nan
+    last_trial_after: float = math.nan
+    last_x_norm: float = math.nan
+    last_y_norm: float = math.nan
+    primal_distance: float = math.nan
+    dual_distance: float = math.nan
 
 
 def run_restart_scheme(avg_count: int, current: Optional[KKTPieces], average: Optional[KKTPieces],
@@ -145,41 +162,56 @@ def run_restart_scheme(avg_count: int, current: Optional[KKTPieces], average: Op
     restart the caller moves the average into the current point (if
     to_average) and resets the average; this function updates the restart
     info as update_last_restart_info! (:733-780) does."""
+    out = RestartDecision(RestartChoice.RESTART_CHOICE_NO_RESTART,
+                          last_trial_before=last_restart_info.kkt_reduction_ratio_last_trial)
     if not avg_count > 0:
-        return RestartDecision(RestartChoice.RESTART_CHOICE_NO_RESTART)
+        out.last_trial_after = last_restart_info.kkt_reduction_ratio_last_trial
+        return out
     restart_length = avg_count
     do_restart = False
     if restart_length >= restart_params.artificial_restart_threshold * iterations_completed:
         do_restart = True
+        out.artificial = True
     if restart_params.restart_scheme == NO_RESTARTS:
         reset_to_average = False
         candidate = None
     else:
         kc = current.weighted_kkt(primal_weight)
         ka = average.weighted_kkt(primal_weight)
+        out.kkt_current, out.kkt_average = kc, ka
         reset_to_average = should_reset_to_average(kc, ka, restart_params.restart_to_current_metric)
         candidate = (average, ka) if reset_to_average else (current, kc)
+    out.reset_to_average = reset_to_average
     if not do_restart:
         if restart_params.restart_scheme == ADAPTIVE_KKT:
+            out.kkt_last = last_restart_info.point.weighted_kkt(primal_weight)
             do_restart = should_do_adaptive_restart_kkt(candidate[1], restart_params, last_restart_info,
                                                         primal_weight)
+            out.adaptive_checked = True
+            out.ratio = last_restart_info.kkt_reduction_ratio_last_trial
         elif (restart_params.restart_scheme == FIXED_FREQUENCY
               and restart_params.restart_frequency_if_fixed <= restart_length):
             do_restart = True
+    out.last_trial_after = last_restart_info.kkt_reduction_ratio_last_trial
     if not do_restart:
-        return RestartDecision(RestartChoice.RESTART_CHOICE_NO_RESTART)
+        return out
     new_point = average if reset_to_average else current
     # update_last_restart_info!: avg - last = -last after the reset (:753-767)
     lp = last_restart_info.point
+    out.last_x_norm, out.last_y_norm = lp.x_norm, lp.y_norm
     last_restart_info.primal_distance_moved_last_restart_period = (
         math.sqrt(primal_norm_params) * lp.x_norm / math.sqrt(primal_weight))
     last_restart_info.dual_distance_moved_last_restart_period = (
         math.sqrt(dual_norm_params) * lp.y_norm * math.sqrt(primal_weight))
+    out.primal_distance = last_restart_info.primal_distance_moved_last_restart_period
+    out.dual_distance = last_restart_info.dual_distance_moved_last_restart_period
     last_restart_info.point = new_point
     last_restart_info.last_restart_length = restart_length
     if reset_to_average:
-        return RestartDecision(RestartChoice.RESTART_CHOICE_RESTART_TO_AVERAGE, True)
-    return RestartDecision(RestartChoice.RESTART_CHOICE_WEIGHTED_AVERAGE_RESET, False)
+        out.choice, out.to_average = RestartChoice.RESTART_CHOICE_RESTART_TO_AVERAGE, True
+    else:
+        out.choice = RestartChoice.RESTART_CHOICE_WEIGHTED_AVERAGE_RESET
+    return out
 
 
 def compute_new_primal_weight(last_restart_info: RestartInfo, primal_weight: float,

@@ -204,8 +204,9 @@ class InstanceController:
 
     KKT_PASSES_PER_TERMINATION_EVALUATION = 2.0
 
-    def __init__(self, params: PdhgParameters, ot_cache: CachedOTInfo):
+    def __init__(self, params: PdhgParameters, ot_cache: CachedOTInfo, restart_trace: Optional[list] = None):
         self.params = params
+        self.restart_trace = restart_trace
         self.ot_cache = ot_cache
         crit = params.termination_criteria
         self.eps_ratio = crit.eps_optimal_absolute / crit.eps_optimal_relative
@@ -244,6 +245,7 @@ class InstanceController:
         (decision, new primal weight)."""
         primal_norm_params, dual_norm_params = define_norms(st.step_size, st.primal_weight)
         rp = self.params.restart_params
+        last_point = self.last_restart_info.point
         decision = run_restart_scheme(int(st.avg_count), KKTPieces.from_point_stats(cur),
                                       KKTPieces.from_point_stats(avg) if st.avg_count > 0 else None,
                                       self.last_restart_info, iteration - 1, primal_norm_params, dual_norm_params,
@@ -254,20 +256,47 @@ class InstanceController:
             self.restarts += 1
             weight = compute_new_primal_weight(self.last_restart_info, st.primal_weight,
                                                rp.primal_weight_update_smoothing, self.params.verbosity)
+        if self.restart_trace is not None:
+            self.restart_trace.append(restart_record(decision, cur, avg, last_point, st, iteration, weight))
         return decision, weight
+
+
+def _pieces_dict(pieces: Optional[KKTPieces]) -> dict:
+    keys = ("l2_primal_residual", "l2_dual_residual", "primal_objective", "dual_objective", "x_norm", "y_norm")
+    if pieces is None:
+        return {k: math.nan for k in keys}
+    return {k: getattr(pieces, k) for k in keys}
+
+
+def restart_record(decision, cur, avg, last_point: KKTPieces, st, iteration: int, weight_after: float) -> dict:
+    """One evaluation's restart decision with everything it read, for logs and
+    tests (the fields of the oracle's restart trace)."""
+    return {"iteration": int(iteration), "restart_used": int(decision.choice), "artificial": int(decision.artificial),
+            "adaptive_checked": int(decision.adaptive_checked), "reset_to_average": int(decision.reset_to_average),
+            "avg_count": int(st.avg_count), "iterations_completed": int(iteration) - 1,
+            "step_size": st.step_size, "primal_weight_before": st.primal_weight, "primal_weight_after": weight_after,
+            "kkt_current": decision.kkt_current, "kkt_average": decision.kkt_average,
+            "kkt_last": decision.kkt_last, "ratio": decision.ratio,
+            "last_trial_before": decision.last_trial_before, "last_trial_after": decision.last_trial_after,
+            "last_x_norm": decision.last_x_norm, "last_y_norm": decision.last_y_norm,
+            "primal_distance": decision.primal_distance, "dual_distance": decision.dual_distance,
+            "current": _pieces_dict(KKTPieces.from_point_stats(cur)),
+            "average": _pieces_dict(KKTPieces.from_point_stats(avg) if st.avg_count > 0 else None),
+            "last": _pieces_dict(last_point)}
 
 
 def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, *,
              storage: int = _lib.STORAGE_F64, device: int = 0, warm_start=None,
              solver: Optional[_lib.DeviceSolver] = None, profile: Optional[SolveProfile] = None,
-             use_graphs: bool = False) -> SaddlePointOutput:
+             use_graphs: bool = False, restart_trace: Optional[list] = None) -> SaddlePointOutput:
     """PDOT.optimize (solver_gpu.jl:413-720) on an MI355X.
 
     storage: _lib.STORAGE_F64 (the reference's precision) or
     _lib.STORAGE_F32 (C, X and the average stored in FP32; duals, sums and
     scalars in FP64).  warm_start: optional (u0, v0) initial duals; X starts at
     0 and G at (C - u0) - v0.  solver: reuse a DeviceSolver whose problem is
-    already uploaded (the upload is then skipped)."""
+    already uploaded (the upload is then skipped).  restart_trace: a list
+    that receives one record per non-terminating evaluation (restart_record)."""
     from .io import PointCloudProblem
     points = isinstance(original_problem, PointCloudProblem)
     nrow, ncol = original_problem.num_row, original_problem.num_col
@@ -297,7 +326,7 @@ def optimize(params: PdhgParameters, original_problem: OptimalTransportProblem, 
         u0, v0 = warm_start
     solver.set_start(u0, v0, policy, re, ge, step_size, primal_weight)
 
-    ctl = InstanceController(params, ot_cache)
+    ctl = InstanceController(params, ot_cache, restart_trace)
     start_time = time.time()
     time_basic = 0.0
     iteration_limit = params.termination_criteria.iteration_limit
