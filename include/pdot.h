@@ -181,6 +181,23 @@ int pdot_round(pdot_ctx* ctx, int32_t which, double* X_rounded);
 int pdot_round_matrix(int32_t device, int64_t n, int64_t m, const double* a, const double* b, double* X,
                       char* errbuf, int64_t errlen);
 
+/* No counterpart in the reference: the rounding of pdot_round_matrix for a
+ * plan already in device memory, for callers that cannot allocate device
+ * memory (tight serving loops, code that must not call hipMalloc).  It calls
+ * no hipMalloc, hipFree or other allocating routine: its workspace (row and
+ * column partial sums, scale and error vectors) is a static device array of
+ * 64 MiB per GPU, held until the process ends, which covers plans up to about
+ * 30000 x 30000 on a 256-CU device; larger plans are rejected with
+ * PDOT_E_ARG.  X (ldx >= n), Y (ldy >= n), a and b are FP64 device pointers
+ * of `device` whose allocations hold the given sizes (checked); Y may be X
+ * (in place, ldy == ldx) but must not otherwise overlap it.  The kernels run
+ * on `stream` (a hipStream_t of `device`, NULL for the null stream), and the
+ * call returns once it has synchronised that stream.  Calls on one GPU share
+ * the workspace: calls that overlap in time on the same GPU race and must be
+ * serialised by the caller. */
+int pdot_round_device_noalloc(int32_t device, void* stream, int64_t n, int64_t m, const double* a, const double* b,
+                              const double* X, int64_t ldx, double* Y, int64_t ldy, char* errbuf, int64_t errlen);
+
 /* Host-side text output, write_vector_to_file of scripts/solve.jl:5-11: one
  * element per line (a matrix in column-major order), each as Julia's println
  * prints a Float64 (shortest round-trip digits; "0.0001", "123456.0",

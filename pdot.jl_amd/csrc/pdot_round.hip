@@ -23,6 +23,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <string>
 
 #include "pdot.h"
@@ -148,8 +149,44 @@ hipError_t launch_pass(int mode, hipStream_t stream, const RoundArgs& p) {
 
 }  // namespace
 
+namespace {
+
+// Unit geometry of an n x m rounding on the current device: NG row groups of
+// kUnitRows rows, CB column blocks of TC columns; about 16 waves per CU of
+// work units, each streaming at least 16 columns.
+struct RoundGeometry {
+    int64_t NG, CB, TC, words;
+    bool fits;
+};
+
+RoundGeometry round_geometry(int64_t n, int64_t m) {
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    RoundGeometry g;
+    g.NG = (n + kUnitRows - 1) / kUnitRows;
+    int64_t CB = ((int64_t)cus * 16 + g.NG - 1) / g.NG;
+    const int64_t max_cb = (m + 15) / 16;
+    if (CB > max_cb) CB = max_cb;
+    if (CB < 1) CB = 1;
+    g.TC = (m + CB - 1) / CB;
+    g.CB = (m + g.TC - 1) / g.TC;
+    g.fits = g.NG * g.CB / kUnitsPerBlock + 1 <= 0x7fffffffLL;
+    g.words = g.CB * n + g.NG * m + 2 * n + 2 * m;
+    return g;
+}
+
+}  // namespace
+
+int64_t pdot_round_work_words(int64_t n, int64_t m) {
+    if (n < 1 || m < 1) return -1;
+    const RoundGeometry g = round_geometry(n, m);
+    return g.fits ? g.words : -1;
+}
+
 int pdot_round_plan(hipStream_t stream, const void* X, int32_t storage, int64_t ld, int64_t n, int64_t m,
-                    const double* a, const double* b, double* Y, int64_t ldy, std::string* err) {
+                    const double* a, const double* b, double* Y, int64_t ldy, std::string* err, double* work,
+                    int64_t work_words) {
     auto failed = [&](int code, const std::string& msg) {
         if (err) *err = msg;
         return code;
@@ -160,25 +197,32 @@ int pdot_round_plan(hipStream_t stream, const void* X, int32_t storage, int64_t 
         return failed(PDOT_E_ARG, "pdot_round_plan: unknown storage");
     const bool f32 = storage == PDOT_STORAGE_F32;
     if (f32 && static_cast<const void*>(Y) == X) return failed(PDOT_E_ARG, "pdot_round_plan: FP32 plan cannot be rounded in place");
+    if (static_cast<const void*>(Y) == X && ldy != ld)
+        return failed(PDOT_E_ARG, "pdot_round_plan: in-place rounding needs ldy == ld");
+    if (static_cast<const void*>(Y) != X) {
+        // a distinct output must not overlap the plan
+        const char* x0 = static_cast<const char*>(X);
+        const char* y0 = reinterpret_cast<const char*>(Y);
+        const int64_t xb = (ld * (m - 1) + n) * (f32 ? 4 : 8), yb = (ldy * (m - 1) + n) * 8;
+        if (y0 < x0 + xb && x0 < y0 + yb) return failed(PDOT_E_ARG, "pdot_round_plan: output overlaps the plan");
+    }
 
-    // Unit geometry: NG row groups of kUnitRows rows, CB column blocks; about
-    // 16 waves per CU of work units, each streaming at least 16 columns.
-    int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const int64_t NG = (n + kUnitRows - 1) / kUnitRows;
-    int64_t CB = ((int64_t)cus * 16 + NG - 1) / NG;
-    const int64_t max_cb = (m + 15) / 16;
-    if (CB > max_cb) CB = max_cb;
-    if (CB < 1) CB = 1;
-    const int64_t TC = (m + CB - 1) / CB;
-    CB = (m + TC - 1) / TC;
-    if (NG * CB / kUnitsPerBlock + 1 > 0x7fffffffLL) return failed(PDOT_E_ARG, "pdot_round_plan: grid too large");
+    const RoundGeometry geo = round_geometry(n, m);
+    if (!geo.fits) return failed(PDOT_E_ARG, "pdot_round_plan: grid too large");
+    const int64_t NG = geo.NG, CB = geo.CB, TC = geo.TC;
 
-    double* buf = nullptr;
-    const size_t words = (size_t)(CB * n + NG * m + 2 * n + 2 * m);
-    hipError_t e = hipMalloc((void**)&buf, sizeof(double) * words);
-    if (e != hipSuccess) return failed(PDOT_E_NOMEM, std::string("pdot_round_plan: hipMalloc: ") + hipGetErrorString(e));
+    double* buf = work;
+    const size_t words = (size_t)geo.words;
+    hipError_t e = hipSuccess;
+    if (work) {
+        if (work_words < geo.words)
+            return failed(PDOT_E_ARG, "pdot_round_plan: workspace of " + std::to_string(work_words) +
+                                          " doubles is smaller than the " + std::to_string(geo.words) + " needed");
+    } else {
+        e = hipMalloc((void**)&buf, sizeof(double) * words);
+        if (e != hipSuccess)
+            return failed(PDOT_E_NOMEM, std::string("pdot_round_plan: hipMalloc: ") + hipGetErrorString(e));
+    }
     RoundArgs p;
     p.X = X;
     p.rowpart = buf;
@@ -220,7 +264,7 @@ int pdot_round_plan(hipStream_t stream, const void* X, int32_t storage, int64_t 
         msg = std::string("pdot_round_plan: ") + hipGetErrorString(e);
         hipStreamSynchronize(stream);
     }
-    hipFree(buf);
+    if (!work) hipFree(buf);
     if (rc != PDOT_OK) return failed(rc, msg);
     return PDOT_OK;
 }
@@ -274,5 +318,103 @@ extern "C" int pdot_round_matrix(int32_t device, int64_t n, int64_t m, const dou
     if (da) hipFree(da);
     if (db) hipFree(db);
     if (stream) hipStreamDestroy(stream);
+    return done(rc);
+}
+
+// ------------------------------------------------ allocation-free variant
+
+// Workspace of pdot_round_device_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  64 MiB covers the row and column
+// partial sums of plans up to about 30000 x 30000 on a 256-CU device.
+constexpr int64_t kNoallocScratchBytes = 64ll << 20;
+constexpr int64_t kNoallocScratchWords = kNoallocScratchBytes / (int64_t)sizeof(double);
+static __device__ __align__(256) double scratch[kNoallocScratchWords];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// The device address of this device's copy of `scratch` (the current device
+// must be `device`).  In host code the array's name is only a host-side
+// stand-in; kernels get this address.
+int scratch_address(int device, double** out, std::string* err) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr) {
+            *err = std::string("pdot_round_device_noalloc: hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return PDOT_E_HIP;
+        }
+        g_scratch_ptr[device] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return PDOT_OK;
+}
+
+// A device pointer of `device` whose allocation holds [p, p + bytes).
+bool device_range_ok(const void* p, int64_t bytes, int device) {
+    hipPointerAttribute_t attr;
+    if (hipPointerGetAttributes(&attr, p) != hipSuccess) return false;
+    if (attr.type != hipMemoryTypeDevice || attr.device != device) return false;
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (hipMemGetAddressRange(&base, &size, const_cast<void*>(p)) != hipSuccess) return false;
+    const char* b = static_cast<const char*>(base);
+    const char* q = static_cast<const char*>(p);
+    return q >= b && bytes >= 0 && (uint64_t)(q - b) + (uint64_t)bytes <= (uint64_t)size;
+}
+
+}  // namespace
+
+extern "C" int pdot_round_device_noalloc(int32_t device, void* stream, int64_t n, int64_t m, const double* a,
+                                         const double* b, const double* X, int64_t ldx, double* Y, int64_t ldy,
+                                         char* errbuf, int64_t errlen) {
+    std::string err;
+    auto done = [&](int code) {
+        if (errbuf && errlen > 0) {
+            const size_t k = err.size() < (size_t)(errlen - 1) ? err.size() : (size_t)(errlen - 1);
+            err.copy(errbuf, k);
+            errbuf[k] = '\0';
+        }
+        return code;
+    };
+    if (n < 1 || m < 1 || ldx < n || ldy < n || !a || !b || !X || !Y) {
+        err = "pdot_round_device_noalloc: invalid arguments";
+        return done(PDOT_E_ARG);
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count || device >= kMaxDevices) {
+        err = "pdot_round_device_noalloc: no such device";
+        return done(PDOT_E_ARG);
+    }
+    hipError_t e = hipSetDevice(device);
+    if (e != hipSuccess) {
+        err = std::string("pdot_round_device_noalloc: hipSetDevice: ") + hipGetErrorString(e);
+        return done(PDOT_E_HIP);
+    }
+    double* work = nullptr;
+    int rc = scratch_address(device, &work, &err);
+    if (rc != PDOT_OK) return done(rc);
+    const int64_t words = pdot_round_work_words(n, m);
+    if (words < 0 || words > kNoallocScratchWords) {
+        err = "pdot_round_device_noalloc: a " + std::to_string(n) + " x " + std::to_string(m) + " plan needs " +
+              std::to_string(words < 0 ? -1 : words * 8) + " bytes of workspace, above the maximum of " +
+              std::to_string(kNoallocScratchBytes);
+        return done(PDOT_E_ARG);
+    }
+    const int64_t xbytes = (ldx * (m - 1) + n) * 8, ybytes = (ldy * (m - 1) + n) * 8;
+    const bool ok = device_range_ok(X, xbytes, device) && device_range_ok(Y, ybytes, device) &&
+                    device_range_ok(a, n * 8, device) && device_range_ok(b, m * 8, device);
+    (void)hipGetLastError();  // a failed attribute query leaves no error for the launches below
+    if (!ok) {
+        err = "pdot_round_device_noalloc: X, Y, a and b must be device memory of this device holding the given sizes";
+        return done(PDOT_E_ARG);
+    }
+    rc = pdot_round_plan(static_cast<hipStream_t>(stream), X, PDOT_STORAGE_F64, ldx, n, m, a, b, Y, ldy, &err, work,
+                         kNoallocScratchWords);
+    if (rc != PDOT_OK && err.rfind("pdot_round_plan", 0) == 0) err = "pdot_round_device_noalloc: " + err.substr(17);
     return done(rc);
 }

@@ -118,6 +118,7 @@ SIGNATURES = {
     # rounding and output
     "pdot_round": (ctypes.c_int, [_ctx_p, ctypes.c_int32, _c_double_p]),
     "pdot_round_matrix": (ctypes.c_int, [ctypes.c_int32, ctypes.c_int64, ctypes.c_int64, _c_double_p, _c_double_p, _c_double_p, ctypes.c_char_p, ctypes.c_int64]),
+    "pdot_round_device_noalloc": (ctypes.c_int, [ctypes.c_int32, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "pdot_write_julia_vector": (ctypes.c_int, [ctypes.c_char_p, _c_double_p, ctypes.c_int64, ctypes.c_char_p, ctypes.c_int64]),
     "pdot_format_julia_float": (ctypes.c_int, [ctypes.c_double, ctypes.c_char_p, ctypes.c_int64]),
     # batches
