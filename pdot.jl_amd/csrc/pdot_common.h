@@ -428,9 +428,12 @@ constexpr int kMaxDim = 3;
 // of a/s, Markstein's theorem makes the second correction the correctly
 // rounded quotient, the value of the IEEE division a / s.  Zero, tiny, huge
 // and non-finite quotients, and scales whose reciprocal could lose precision,
-// take the division itself.
+// take the division itself, and so do numerators below 2^-960, whose
+// residuals a - q*s could fall below the subnormal granularity 2^-1074 (the
+// relative bound alone underflows when s < 2^-122).
 __device__ __forceinline__ double exact_quotient(double a, double s, double y) {
-    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
+    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-960 && a >= 0x1p-900 * s && a <= 0x1p+900 * s))
+        return a / s;
     const double q0 = a * y;
     const double r0 = fma(-q0, s, a);
     const double q1 = fma(r0, y, q0);

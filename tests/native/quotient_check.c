@@ -8,7 +8,8 @@
 #include <stdint.h>
 
 static double quotient(double a, double s, double y) {
-    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-900 * s && a <= 0x1p+900 * s)) return a / s;
+    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-960 && a >= 0x1p-900 * s && a <= 0x1p+900 * s))
+        return a / s;
     const double q0 = a * y;
     const double r0 = fma(-q0, s, a);
     const double q1 = fma(r0, y, q0);
@@ -38,6 +39,24 @@ long quotient_mismatches_random(double s, long n, uint64_t seed) {
         x ^= x << 13; x ^= x >> 7; x ^= x << 17;
         const double u = (double)(x >> 11) * 0x1p-53;
         const double a = s * u;
+        const double cand[3] = {a, nextafter(a, 0.0), nextafter(a, INFINITY)};
+        for (int c = 0; c < 3; ++c) bad += differs(quotient(cand[c], s, y), cand[c] / s);
+    }
+    return bad;
+}
+
+/* Tiny numerators: a = m * 2^e with m uniform in [1, 2) (53 random bits) and e
+ * uniform in [emin, emax], including subnormal a (e < -1022 rounds m * 2^e to
+ * the subnormal grid), plus each a's neighbours one ulp away. */
+long quotient_mismatches_tiny(double s, long n, int emin, int emax, uint64_t seed) {
+    const double y = 1.0 / s;
+    uint64_t x = seed ? seed : 88172645463325252ULL;
+    long bad = 0;
+    for (long k = 0; k < n; ++k) {
+        x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+        const double mant = 1.0 + (double)(x >> 11) * 0x1p-53;
+        const int e = emin + (int)((x & 0x7ff) % (uint64_t)(emax - emin + 1));
+        const double a = ldexp(mant, e);
         const double cand[3] = {a, nextafter(a, 0.0), nextafter(a, INFINITY)};
         for (int c = 0; c < 3; ++c) bad += differs(quotient(cand[c], s, y), cand[c] / s);
     }

@@ -27,6 +27,9 @@ def _lib(tmp_path):
     lib.quotient_mismatches.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.c_long, ctypes.c_double]
     lib.quotient_mismatches_random.restype = ctypes.c_long
     lib.quotient_mismatches_random.argtypes = [ctypes.c_double, ctypes.c_long, ctypes.c_uint64]
+    lib.quotient_mismatches_tiny.restype = ctypes.c_long
+    lib.quotient_mismatches_tiny.argtypes = [ctypes.c_double, ctypes.c_long, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_uint64]
     return lib
 
 
@@ -56,3 +59,21 @@ def test_quotient_on_instance_costs(tmp_path):
     s = float(d.max())
     a = np.ascontiguousarray(d.ravel())
     assert lib.quotient_mismatches(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(a), s) == 0
+
+
+def test_quotient_tiny_numerators_at_small_scales(tmp_path):
+    """Scales below 2^-122 (where the relative bound a >= 2^-900 s underflows),
+    not powers of two, with subnormal and near-subnormal numerators: the
+    absolute bound a >= 2^-960 sends the numerators whose residuals could be
+    inexact to the division."""
+    lib = _lib(tmp_path)
+    rng = np.random.default_rng(11)
+    scales = [float.fromhex("0x1.bc3d8f9ca9bf6p-167"), 3e-300, 7.0 / 3.0 * 2.0 ** -500, 2.0 ** -123 * 1.5,
+              1e-37 * np.pi] + list(10.0 ** rng.uniform(-300, -37, 20)) + list(10.0 ** rng.uniform(-6, 6, 5))
+    for k, s in enumerate(scales):
+        assert lib.quotient_mismatches_tiny(float(s), 100_000, -1074, -900, 500 + k) == 0, s
+        assert lib.quotient_mismatches_tiny(float(s), 50_000, -970, -950, 900 + k) == 0, s
+    # the case that showed the underflowing guard
+    a = np.array([float.fromhex("0x0.0000000000435p-1022")])
+    s = float.fromhex("0x1.bc3d8f9ca9bf6p-167")
+    assert lib.quotient_mismatches(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), 1, s) == 0
