@@ -28,7 +28,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .io import OptimalTransportProblem, PointCloudProblem
+from .io import OptimalTransportProblem, PointCloudProblem, point_cost_matrix
 
 _GAMMA = np.uint64(0x9E3779B97F4A7C15)
 _M1 = np.uint64(0xBF58476D1CE4E5B9)
@@ -89,19 +89,11 @@ def normalized(w: np.ndarray) -> np.ndarray:
 
 def sqeuclid_cost(x: np.ndarray, y: np.ndarray) -> np.ndarray:
     """C_ij = sum_d (x_id - y_jd)^2, returned column-major (n x m)."""
-    ct = np.zeros((y.shape[0], x.shape[0]), dtype=np.float64)
-    for d in range(x.shape[1]):
-        t = y[:, d][:, None] - x[:, d][None, :]
-        t *= t
-        ct += t
-    return ct.T
+    return point_cost_matrix(x, y, "sqeuclid")
 
 
 def l1_cost(x: np.ndarray, y: np.ndarray) -> np.ndarray:
-    ct = np.zeros((y.shape[0], x.shape[0]), dtype=np.float64)
-    for d in range(x.shape[1]):
-        ct += np.abs(y[:, d][:, None] - x[:, d][None, :])
-    return ct.T
+    return point_cost_matrix(x, y, "l1")
 
 
 def uniform_sqeuclid_points(n: int = 1000, m: Optional[int] = None, seed: int = 1000) -> PointCloudProblem:

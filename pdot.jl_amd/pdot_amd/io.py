@@ -2,6 +2,8 @@
 gpu_to_cpu!, io.jl:18-53) is DeviceSolver.upload/download in _lib.py."""
 from __future__ import annotations
 
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass
 
 import numpy as np
@@ -77,12 +79,40 @@ class PointCloudProblem:
         return self.target_points.shape[0]
 
     def dense(self) -> OptimalTransportProblem:
-        x, y = self.source_points, self.target_points
-        ct = np.zeros((y.shape[0], x.shape[0]), dtype=np.float64)
-        for d in range(x.shape[1]):
-            t = y[:, d][:, None] - x[:, d][None, :]
-            ct += t * t if self.cost == "sqeuclid" else np.abs(t)
-        C = ct.T
+        C = point_cost_matrix(self.source_points, self.target_points, self.cost)
         if self.normalize:
-            C = C / C.max()
+            s = float(C.max())
+            _parallel_columns(C.shape[1], lambda j0, j1: np.divide(C[:, j0:j1], s, out=C[:, j0:j1]))
         return OptimalTransportProblem(C, self.source_distribution, self.target_distribution)
+
+
+def _parallel_columns(m: int, fn, min_cols: int = 64) -> None:
+    """fn(j0, j1) over column ranges on host threads (numpy releases the GIL)."""
+    workers = max(1, min(16, os.cpu_count() or 1, m // min_cols))
+    if workers == 1:
+        fn(0, m)
+        return
+    step = -(-m // (4 * workers))
+    with ThreadPoolExecutor(workers) as ex:
+        for f in [ex.submit(fn, j0, min(m, j0 + step)) for j0 in range(0, m, step)]:
+            f.result()
+
+
+def point_cost_matrix(x: np.ndarray, y: np.ndarray, cost: str) -> np.ndarray:
+    """C_ij = sum_d f(y_jd - x_id), f(t) = t*t ("sqeuclid") or |t| ("l1"),
+    summed from 0 in d order; column-major n x m, formed on host threads
+    (each element by the same operations, so the result does not depend on the
+    thread count)."""
+    n, m = x.shape[0], y.shape[0]
+    ct = np.zeros((m, n), dtype=np.float64)
+
+    def rows(j0, j1):
+        for d in range(x.shape[1]):
+            t = y[j0:j1, d][:, None] - x[:, d][None, :]
+            if cost == "sqeuclid":
+                t *= t
+            else:
+                np.abs(t, out=t)
+            ct[j0:j1] += t
+    _parallel_columns(m, rows)
+    return ct.T
