@@ -1,8 +1,8 @@
 """Benchmark of the PDOT hot path on one MI355X.
 
-Workload: the 10000 x 10000 Gaussian-shift transport problem (source N(0,I2),
-target N((2,0),I2), squared Euclidean cost / max, U(0,1) masses, seeded
-synthetic data), solved by restarted adaptive PDHG.
+Headline workload: the 10000 x 10000 Gaussian-shift transport problem
+(source N(0,I2), target N((2,0),I2), squared Euclidean cost / max, U(0,1)
+masses, seeded synthetic data), solved by restarted adaptive PDHG.
 
 A step is one PDHG iteration (one take_step! pass over C, X and the running
 average).  The timed region runs K steps of the FP64 form (the reference's
@@ -10,13 +10,24 @@ precision) on inputs resident in HBM, with HIP events recorded on the
 library's stream around every launch of the streaming step kernel, so the
 same run gives the iteration rate and the step kernel's average duration.
 
-Also reported (extra fields): the same measurement for the FP32-storage form,
-full solves to 1e-4 relative KKT for both forms (time from after the upload
-to the terminating evaluation, objectives against each other), and the CPU
-baseline: the oracle's literal restatement of the reference on the same
-instance, a bounded number of iterations on the host cores.
+Also reported (extra fields): the same measurement for the FP32-storage and
+point-cloud forms, full solves to 1e-4 relative KKT (time from after the
+upload to the terminating evaluation, objectives against each other), the
+CPU baseline (the oracle's literal restatement of the reference, a bounded
+number of iterations on the host cores), and one entry per other workload,
+each with the roofline of its dominant kernel:
+  c1_cpu_1000        1000 x 1000 uniform points, FP64, solved to 1e-4 on the
+                     host cores by the literal restatement (and on the GPU);
+  c2_f32_1e-6        the headline instance in FP32 storage solved to 1e-6;
+  c3_l1_mixture_30k  30000 x 30000 L1 mixture, dense FP32 C / FP64 duals, and
+                     the FP64 form for the objective error, solved to 1e-4;
+  c4_batched_512     512 instances of n, m in [64, 2048], FP32, to 1e-4, at
+                     the reference's initial primal weight and the calibrated one;
+  c5_warm_start_4096 held-out 4096 x 4096 Gaussian-mixture instances: zero
+                     start against the learned starting states.
+Together about four minutes on one MI355X; --no-extra skips them.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu] [--no-extra]
 """
 from __future__ import annotations
 
@@ -121,61 +132,79 @@ def solve_to_tol(prob, storage, tol, dev=None, do_round=True, primal_importance=
     return res
 
 
-def bench_l1_mixture(args) -> int:
-    """30000 x 30000 L1-cost mixture, FP32 storage (C, X, X_avg 3.6 GB each)
-    with FP64 duals; the FP64 form on the same instance gives the objective
-    error of the FP32-storage result."""
+def roofline(achieved_gbs: float, bytes_per_launch: int, avg_launch_us: float, kernel: str, traffic=None) -> dict:
+    return {"bound": "hbm", "achieved": achieved_gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved_gbs / PEAK_HBM_GBS, "traffic": traffic, "kernel": kernel,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_us}
+
+
+def step_measurement(dev, n: int, m: int, elem: int, steps: int, warmup: int, matrix_streams: int = 4) -> dict:
+    """K timed take_step! calls from the zero start: iteration rate and the
+    step kernel's HIP-event average against its algorithmic bytes."""
+    ks, tt, st = measure_steps(dev, steps, warmup)
+    alg = algorithmic_bytes_per_step(n, m, elem, matrix_streams)
+    k_avg = ks / steps * 1e-3
+    return {"ms_per_step": tt / steps, "iters_per_s": steps / (tt * 1e-3), "step_kernel_us": k_avg * 1e6,
+            "step_kernel_GBs": alg / k_avg / 1e9, "bytes_per_step": alg, "numerical_error": int(st.numerical_error)}
+
+
+def l1_mixture(n: int, steps: int, warmup: int, solve: bool = True, point_cloud: bool = True) -> dict:
+    """n x n L1-cost mixture (c3 at n = 30000): dense FP32 storage (C, X and
+    X_avg 3.6 GB each at 30000) with FP64 duals, and the FP64 form on the same
+    instance for the objective error of the FP32-storage result."""
     from pdot_amd import _lib, instances
-    n = m = 30000 if args.n == 10000 else args.n
+    m = n
     t0 = time.time()
     prob = instances.l1_mixture_3d(n)
     gen = time.time() - t0
-    out = {"metric": "30k x 30k L1 mixture: FP32-storage vs FP64 solve to 1e-4", "unit": "s",
+    out = {"metric": f"{n}x{n} L1 mixture: FP32-storage vs FP64 solve to 1e-4", "unit": "s",
            "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
-           "config": {"workload": f"l1_mixture_3d {n}x{m}", "n": n, "m": m}, "generate_sec": gen}
+           "config": {"workload": f"l1_mixture_3d {n}x{m} (source U[0,1]^3, 4-component mixture target, L1 cost, "
+                                  f"lognormal masses, seed 3000)", "n": n, "m": m}, "generate_sec": gen}
     for storage, name, elem in ((_lib.STORAGE_F32, "f32", 4), (_lib.STORAGE_F64, "f64", 8)):
         dev = _lib.DeviceSolver(n, m, storage=storage)
         t0 = time.time()
         dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
-        up = time.time() - t0
-        ks, tt, st = measure_steps(dev, args.steps, args.warmup)
-        alg = algorithmic_bytes_per_step(n, m, elem)
-        k_avg = ks / args.steps * 1e-3
-        res = {"upload_sec": up, "ms_per_step": tt / args.steps, "step_kernel_GBs": alg / k_avg / 1e9,
-               "device_GB": dev.device_bytes() / 1e9}
-        if not args.no_solve:
+        res = {"upload_sec": time.time() - t0, "device_GB": dev.device_bytes() / 1e9}
+        res.update(step_measurement(dev, n, m, elem, steps, warmup))
+        if solve:
             res["solve"] = solve_to_tol(prob, storage, 1e-4, dev=dev, do_round=False)
         out[name] = res
         dev.close()
-    # point-cloud cost form, FP32 storage: C never stored, 4 n*m streams per step
-    pts = instances.l1_mixture_3d_points(n)
-    dev = _lib.DeviceSolver(n, m, storage=_lib.STORAGE_F32)
-    t0 = time.time()
-    dev.upload_points(pts)
-    up = time.time() - t0
-    ks, tt, st = measure_steps(dev, args.steps, args.warmup)
-    alg = algorithmic_bytes_per_step(n, m, 4, matrix_streams=3)
-    res = {"upload_sec": up, "ms_per_step": tt / args.steps, "step_kernel_GBs": alg / (ks / args.steps * 1e-3) / 1e9,
-           "device_GB": dev.device_bytes() / 1e9}
-    if not args.no_solve:
-        res["solve"] = solve_to_tol(pts, _lib.STORAGE_F32, 1e-4, dev=dev, do_round=False)
-    out["f32_point_cloud"] = res
-    dev.close()
-    out["roofline"] = {"bound": "hbm", "achieved": out["f32"]["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "frac": out["f32"]["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": None,
-                       "kernel": "step_kernel<float> (FP32 storage)",
-                       "bytes_per_launch": algorithmic_bytes_per_step(n, m, 4)}
-    if not args.no_solve:
-        out["value"] = out["f32"]["solve"]["seconds"]
-        out["objective_rel_err_f32_vs_f64"] = (abs(out["f32"]["solve"]["objective"] - out["f64"]["solve"]["objective"])
+    del prob
+    if point_cloud:
+        # point-cloud cost form, FP32 storage: C never stored (3 n*m streams per step on average)
+        pts = instances.l1_mixture_3d_points(n)
+        dev = _lib.DeviceSolver(n, m, storage=_lib.STORAGE_F32)
+        t0 = time.time()
+        dev.upload_points(pts)
+        res = {"upload_sec": time.time() - t0, "device_GB": dev.device_bytes() / 1e9}
+        res.update(step_measurement(dev, n, m, 4, steps, warmup, matrix_streams=3))
+        if solve:
+            res["solve"] = solve_to_tol(pts, _lib.STORAGE_F32, 1e-4, dev=dev, do_round=False)
+        out["f32_point_cloud"] = res
+        dev.close()
+    f32 = out["f32"]
+    out["roofline"] = roofline(f32["step_kernel_GBs"], f32["bytes_per_step"], f32["step_kernel_us"],
+                               "step_kernel<float> (dense FP32 storage)")
+    if solve:
+        out["value"] = f32["solve"]["seconds"]
+        out["speedup_f32_vs_f64"] = out["f64"]["solve"]["seconds"] / f32["solve"]["seconds"]
+        out["objective_rel_err_f32_vs_f64"] = (abs(f32["solve"]["objective"] - out["f64"]["solve"]["objective"])
                                                / abs(out["f64"]["solve"]["objective"]))
-    print(json.dumps(out))
+    return out
+
+
+def bench_l1_mixture(args) -> int:
+    n = 30000 if args.n == 10000 else args.n
+    print(json.dumps(l1_mixture(n, args.steps, args.warmup, solve=not args.no_solve)))
     return 0
 
 
-def bench_batched(args) -> int:
+def batched(steps: int, solve: bool = True) -> dict:
     """512 instances, n and m uniform in [64, 2048], FP32 storage, every
-    instance solved to 1e-4 relative KKT in one batch."""
+    instance solved to 1e-4 relative KKT in one batch, at the reference's
+    initial primal weight (primal_importance 1.0) and at the calibrated one."""
     import numpy as np
     import pdot_amd as P
     from pdot_amd import _lib, instances
@@ -193,51 +222,212 @@ def bench_batched(args) -> int:
     b.set_start(_lib.POLICY_ADAPTIVE, 2.0, 1.0, 1.0, 1.0)
     b.time_steps(1)  # the first pass after a start has an empty average (untimed)
     # an even count: the passes that defer the averages' update alternate with those applying it
-    steps = max(2, args.steps // 8 * 2)
+    steps = max(2, steps // 8 * 2)
     ks, tt = b.time_steps(steps)
     total_elems = sum(n * m for n, m in zip(ns, ms))
     alg = 4 * 4 * total_elems + 80 * (sum(ns) + sum(ms))  # 3 and 5 element streams alternately
     res = {"metric": "512-instance batch (n, m in [64, 2048]) solved to 1e-4, FP32 storage", "unit": "s",
            "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
-           "config": {"workload": "batched_uniform 512 x [64..2048]^2", "instances": 512,
-                      "elements": total_elems, "storage": "f32"},
+           "config": {"workload": "batched_uniform 512 x [64..2048]^2 (U[0,1]^2 points, squared Euclidean, "
+                                  "U(0,1) masses, seed 4000)", "instances": 512, "elements": total_elems,
+                      "storage": "f32"},
            "generate_sec": gen, "upload_sec": up, "autotune": tune["best"],
            "pass_ms_all_active": tt / steps, "step_kernel_GBs": alg / (ks / steps * 1e-3) / 1e9,
            "bytes_per_pass": alg}
-    res["roofline"] = {"bound": "hbm", "achieved": res["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                       "frac": res["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": measured_batch_traffic(),
-                       "kernel": "bstep_kernel<float> (every instance active)", "bytes_per_launch": alg,
-                       "avg_launch_us": ks / steps * 1e3}
-    if not args.no_solve:
-        prof = BatchProfile()
-        outs = optimize_batch(P.default_parameters(1e-4, record_iteration_stats=False), probs, batch=b,
-                              use_graphs=True, profile=prof)
-        iters = np.array([o.iteration_count for o in outs])
-        optimal = sum(o.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL for o in outs)
-        res.update({"value": prof.solve_sec, "solve_sec": prof.solve_sec, "optimal": int(optimal),
-                    "instances_per_s": 512 / prof.solve_sec, "iterations_median": float(np.median(iters)),
-                    "iterations_max": int(iters.max()), "passes_launched": prof.passes_launched,
-                    "evaluations": prof.evaluations})
-        # the same batch with the initial primal weight calibrated for the learned warm start
-        # (primal_importance; the reference's default, used above, is 1.0)
+    res["roofline"] = roofline(res["step_kernel_GBs"], alg, ks / steps * 1e3,
+                               "bstep_kernel<float> (every instance active)", measured_batch_traffic())
+    if solve:
         from dataclasses import replace
         from pdot_amd import learned
+        runs = [("reference_weight", 1.0)]
         w0 = learned.load_initial_primal_weight()
         if w0 is not None:
-            prof2 = BatchProfile()
-            outs2 = optimize_batch(replace(P.default_parameters(1e-4, record_iteration_stats=False),
-                                           primal_importance=w0), probs, batch=b, use_graphs=True, profile=prof2)
-            it2 = np.array([o.iteration_count for o in outs2])
-            obj = np.array([o.iteration_stats[-1].convergence_information[0].primal_objective for o in outs])
-            obj2 = np.array([o.iteration_stats[-1].convergence_information[0].primal_objective for o in outs2])
-            res["with_primal_importance"] = {
-                "primal_importance": w0, "solve_sec": prof2.solve_sec,
-                "optimal": int(sum(o.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
-                                   for o in outs2)),
-                "iterations_median": float(np.median(it2)), "iterations_max": int(it2.max()),
-                "objective_rel_diff_max": float(np.max(np.abs(obj2 - obj) / np.abs(obj)))}
+            runs.append(("calibrated_weight", w0))
+        for name, w in runs:
+            prof = BatchProfile()
+            prm = replace(P.default_parameters(1e-4, record_iteration_stats=False), primal_importance=float(w))
+            outs = optimize_batch(prm, probs, batch=b, use_graphs=True, profile=prof)
+            iters = np.array([o.iteration_count for o in outs])
+            res[name] = {"primal_importance": w, "solve_sec": prof.solve_sec,
+                         "optimal": int(sum(o.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+                                            for o in outs)),
+                         "instances_per_s": 512 / prof.solve_sec, "iterations_median": float(np.median(iters)),
+                         "iterations_max": int(iters.max()), "passes_launched": prof.passes_launched,
+                         "evaluations": prof.evaluations,
+                         "objectives": [o.iteration_stats[-1].convergence_information[0].primal_objective
+                                        for o in outs]}
+        res["value"] = res["reference_weight"]["solve_sec"]
+        if "calibrated_weight" in res:
+            o1 = np.array(res["reference_weight"]["objectives"])
+            o2 = np.array(res["calibrated_weight"]["objectives"])
+            res["objective_rel_diff_max"] = float(np.max(np.abs(o2 - o1) / np.abs(o1)))
+        for name, _ in runs:
+            del res[name]["objectives"]
     b.close()
-    print(json.dumps(res))
+    return res
+
+
+def bench_batched(args) -> int:
+    print(json.dumps(batched(args.steps, solve=not args.no_solve)))
+    return 0
+
+
+def cpu_c1(tol: float = 1e-4) -> dict:
+    """Config c1: 1000 x 1000 uniform points, squared Euclidean, FP64, solved
+    to 1e-4 by the reference algorithm restated on host arrays (the oracle's
+    literal mode, OpenMP over the host cores), and by the GPU on the same
+    instance for comparison."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    import pdot_amd as P
+    from pdot_amd import _lib, instances
+    prob = instances.uniform_sqeuclid(1000, seed=1000)
+    params = P.default_parameters(tol, record_iteration_stats=False)
+    t0 = time.time()
+    r = O.optimize(params, prob, mode=O.LITERAL)
+    wall = time.time() - t0
+    st = r["stats"][-1]
+    out = {"metric": "1000x1000 FP64 solve to 1e-4 on the host cores (reference algorithm, host arrays)",
+           "unit": "s", "higher_is_better": False, "data": "synthetic",
+           "config": {"workload": "uniform_sqeuclid 1000x1000 (U[0,1]^2 points, squared Euclidean, U(0,1) masses, "
+                                  "seed 1000)", "n": 1000, "m": 1000, "storage": "f64"},
+           "value": st["cumulative_time_sec"], "cores": O.num_threads(), "kind": "port",
+           "iterations": r["iteration_count"], "termination_reason": r["termination_reason"],
+           "iters_per_s": int(r["take_step_calls"]) / st["cumulative_time_sec"], "wall_sec": wall,
+           "objective": st["primal_objective"],
+           "note": "the 8 MB arrays are cache-resident on the host; time from after setup to the terminating "
+                   "evaluation, as the reference's cumulative_time_sec"}
+    dev = _lib.DeviceSolver(1000, 1000, storage=_lib.STORAGE_F64)
+    dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+    g = step_measurement(dev, 1000, 1000, 8, 200, 20)
+    g["solve"] = solve_to_tol(prob, _lib.STORAGE_F64, tol, dev=dev, do_round=False)
+    dev.close()
+    out["gpu_f64"] = g
+    out["speedup_gpu_vs_cpu"] = out["value"] / g["solve"]["seconds"]
+    out["roofline"] = roofline(g["step_kernel_GBs"], g["bytes_per_step"], g["step_kernel_us"],
+                               "step_kernel<double> at 1000x1000 (24 MB per pass: cache-resident, launch-bound)")
+    return out
+
+
+def tight_solve(prob, storage, tol: float) -> dict:
+    """One solve of the headline instance to a tighter tolerance."""
+    from pdot_amd import _lib
+    dev = _lib.DeviceSolver(prob.num_row, prob.num_col, storage=storage)
+    dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
+    res = solve_to_tol(prob, storage, tol, dev=dev, do_round=False)
+    dev.close()
+    return res
+
+
+def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_factor: int = 8) -> dict:
+    """Held-out 4096 x 4096 Gaussian-mixture instances (FP32 storage), time
+    and iterations to 1e-4 from each starting state:
+      default_weight      zero start, the reference's initial primal weight 1.0;
+      calibrated_weight   zero start, the initial primal weight calibrated on
+                          solves of the family (a constant);
+      network_weight      zero start, the network's per-instance initial weight
+                          (when the weights file carries that head);
+      network_duals       the network's predicted duals as the starting iterate
+                          (forced, no acceptance test) with the calibrated weight;
+      learned_start       learned.solve_warm: everything the learned state offers,
+                          behind its KKT acceptance test and fallback.
+    Prediction time is included in the learned rows.  A row's solve is capped
+    at limit_factor times the zero start's iterations."""
+    import numpy as np
+    import pdot_amd as P
+    from pdot_amd import _lib, instances, learned
+    net = learned.WarmStartNet(learned.load_params())
+    w_cal = learned.load_initial_primal_weight()
+    has_head = learned.has_weight_head()
+    rows = []
+    for k in range(count):
+        inst = instances.gmm_dirichlet(4096, seed=900200 + k)
+        p = inst.problem
+        n, m = p.cost_matrix.shape
+        dev = _lib.DeviceSolver(n, m, storage=_lib.STORAGE_F32)
+        dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
+        row = {"seed": 900200 + k}
+
+        def run(name, weight, duals=None, extra_sec=0.0, cap=None):
+            from dataclasses import replace
+            prm = replace(P.default_parameters(1e-4, record_iteration_stats=False), primal_importance=float(weight))
+            if cap is not None:
+                prm.termination_criteria.iteration_limit = int(cap)
+            r = solve_to_tol_params(p, _lib.STORAGE_F32, prm, dev, warm_start=duals)
+            r["seconds"] += extra_sec
+            r["primal_importance"] = float(weight)
+            row[name] = r
+            return r
+
+        z = run("default_weight", 1.0)
+        cap = limit_factor * max(z["iterations"], 128)
+        if w_cal is not None:
+            run("calibrated_weight", w_cal, cap=cap)
+        t0 = time.time()
+        u0, v0 = net.predict_duals(inst.x, inst.y, p.source_distribution, p.target_distribution)
+        t_pred = time.time() - t0
+        if has_head:
+            t0 = time.time()
+            w_net = net.predict_weight(inst.x, inst.y, p.source_distribution, p.target_distribution)
+            run("network_weight", w_net, extra_sec=time.time() - t0, cap=cap)
+        if w_cal is not None:
+            run("network_duals", w_cal, duals=(u0, v0), extra_sec=t_pred, cap=cap)
+        dev.close()
+        t0 = time.time()
+        out, info = learned.solve_warm(P.default_parameters(1e-4, record_iteration_stats=False), p, inst.x, inst.y,
+                                       net, storage=_lib.STORAGE_F32)
+        ci = out.iteration_stats[-1].convergence_information[0]
+        row["learned_start"] = {"seconds": time.time() - t0, "iterations": out.iteration_count,
+                                "termination": out.termination_string, "objective": ci.primal_objective,
+                                "duals_used": info.used_duals, "initial_primal_weight": info.primal_weight,
+                                "fell_back": info.fell_back}
+        rows.append(row)
+        print(f"# c5 instance {k}: " + ", ".join(f"{key} {row[key]['iterations']} it" for key in row
+                                                 if key != "seed"), file=sys.stderr, flush=True)
+    net.close()
+    names = [k for k in rows[0] if k != "seed"]
+    summary = {}
+    for name in names:
+        sec = sum(r[name]["seconds"] for r in rows)
+        its = [r[name]["iterations"] for r in rows]
+        summary[name] = {"seconds_total": sec, "iterations_total": int(sum(its)),
+                         "optimal": int(sum(r[name]["termination"] == "OPTIMAL" for r in rows)),
+                         "speedup_vs_default": sum(r["default_weight"]["seconds"] for r in rows) / sec,
+                         "iteration_ratio_median_vs_default": float(np.median(
+                             [r["default_weight"]["iterations"] / max(1, r[name]["iterations"]) for r in rows]))}
+        if "calibrated_weight" in rows[0]:
+            summary[name]["iteration_ratio_median_vs_calibrated"] = float(np.median(
+                [r["calibrated_weight"]["iterations"] / max(1, r[name]["iterations"]) for r in rows]))
+    # roofline of the dominant kernel on this workload (FP32 step kernel at 4096 x 4096; its 201 MB
+    # working set fits in the 256 MB Infinity Cache, so the rate can exceed the HBM figure)
+    inst = instances.gmm_dirichlet(4096, seed=900200)
+    dev = _lib.DeviceSolver(4096, 4096, storage=_lib.STORAGE_F32)
+    dev.upload(inst.problem.cost_matrix, inst.problem.source_distribution, inst.problem.target_distribution)
+    g = step_measurement(dev, 4096, 4096, 4, steps, warmup)
+    dev.close()
+    return {"metric": "learned warm start: time-to-1e-4 on held-out 4096x4096 GMM instances (FP32 storage)",
+            "unit": "s", "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
+            "config": {"workload": "gmm_dirichlet 4096x4096 (3-8 component 2-D mixtures, Dirichlet masses, "
+                                   "squared Euclidean), held-out seeds 900200+", "instances": len(rows),
+                       "storage": "f32"},
+            "value": summary.get("learned_start", {}).get("seconds_total"), "summary": summary,
+            "calibrated_weight": w_cal, "step_kernel": g,
+            "roofline": roofline(g["step_kernel_GBs"], g["bytes_per_step"], g["step_kernel_us"],
+                                 "step_kernel<float> at 4096x4096 (working set fits in the Infinity Cache)"),
+            "rows": rows}
+
+
+def solve_to_tol_params(prob, storage, params, dev, warm_start=None) -> dict:
+    import pdot_amd as P
+    prof = P.SolveProfile()
+    out = P.optimize(params, prob, storage=storage, solver=dev, profile=prof, warm_start=warm_start)
+    ci = out.iteration_stats[-1].convergence_information[0]
+    return {"seconds": prof.solve_sec, "iterations": out.iteration_count, "termination": out.termination_string,
+            "objective": ci.primal_objective, "rel_kkt": P.relative_kkt_error(ci)}
+
+
+def bench_warm_start(args) -> int:
+    print(json.dumps(warm_start_table(args.instances)))
     return 0
 
 
@@ -269,47 +459,31 @@ def bench_precision(args) -> int:
     return 0
 
 
-def bench_warm_start(args) -> int:
-    """Held-out 4096 x 4096 Gaussian-mixture instances (FP32 storage): time to
-    1e-4 relative KKT from the zero start and through the learned warm start
-    (committed network weights, KKT acceptance test and zero-start fallback), the
-    prediction time included."""
-    import numpy as np
-    import pdot_amd as P
-    from pdot_amd import _lib, instances, learned
-    net = learned.WarmStartNet(learned.load_params())
-    rows = []
-    for k in range(args.instances):
-        inst = instances.gmm_dirichlet(4096, seed=900200 + k)
-        p = inst.problem
-        prm = P.default_parameters(1e-4, record_iteration_stats=False)
+def extras(args, prob, f32_roofline) -> dict:
+    """The other workloads' entries of the default run, each printed to stderr
+    as it finishes."""
+    from pdot_amd import _lib
+    out = {}
+
+    def step(name, fn):
         t0 = time.time()
-        zero = P.optimize(prm, p, storage=_lib.STORAGE_F32)
-        tz = time.time() - t0
-        t0 = time.time()
-        warm, info = learned.solve_warm(prm, p, inst.x, inst.y, net, storage=_lib.STORAGE_F32)
-        tw = time.time() - t0
-        rows.append({"seed": 900200 + k, "zero_iterations": zero.iteration_count, "zero_sec": tz,
-                     "warm_iterations": warm.iteration_count, "warm_sec": tw, "duals_used": info.used_duals,
-                     "initial_primal_weight": info.primal_weight, "fell_back": info.fell_back,
-                     "predict_sec": info.predict_sec, "warm_termination": warm.termination_string,
-                     "objective_rel_diff": abs(warm.iteration_stats[-1].convergence_information[0].primal_objective
-                                               - zero.iteration_stats[-1].convergence_information[0].primal_objective)
-                                           / abs(zero.iteration_stats[-1].convergence_information[0].primal_objective)})
-        print(f"# instance {k}: zero {zero.iteration_count} it {tz:.2f}s, warm {warm.iteration_count} it {tw:.2f}s",
-              file=sys.stderr, flush=True)
-    net.close()
-    tz = sum(r["zero_sec"] for r in rows)
-    tw = sum(r["warm_sec"] for r in rows)
-    print(json.dumps({"metric": "learned warm start: time-to-1e-4 on held-out 4096x4096 GMM instances (FP32 storage)",
-                      "value": tz / tw, "unit": "speedup (zero-start time / warm-start time, incl. prediction)",
-                      "higher_is_better": True, "data": "synthetic", "n_gpus": 1,
-                      "config": {"workload": "gmm_dirichlet 4096x4096", "instances": len(rows), "storage": "f32"},
-                      "zero_sec_total": tz, "warm_sec_total": tw,
-                      "iteration_ratio_median": float(np.median([r["zero_iterations"] / max(1, r["warm_iterations"])
-                                                                 for r in rows])),
-                      "rows": rows}))
-    return 0
+        out[name] = fn()
+        out[name]["wall_sec"] = time.time() - t0
+        print(f"# {name} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+
+    if not args.no_cpu:
+        step("c1_cpu_1000", cpu_c1)
+
+    def c2():
+        r = tight_solve(prob, _lib.STORAGE_F32, 1e-6)
+        r.update({"metric": "headline instance, FP32 storage, time to 1e-6 relative KKT", "unit": "s",
+                  "tolerance": 1e-6, "value": r["seconds"], "roofline": f32_roofline})
+        return r
+    step("c2_f32_1e-6", c2)
+    step("c3_l1_mixture_30k", lambda: l1_mixture(30000, max(10, args.steps // 2), args.warmup, point_cloud=False))
+    step("c4_batched_512", lambda: batched(args.steps))
+    step("c5_warm_start_4096", lambda: warm_start_table(args.instances))
+    return out
 
 
 def main() -> int:
@@ -318,17 +492,19 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=10000)
-    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves")
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves (and the extras)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
+    ap.add_argument("--no-extra", action="store_true", help="skip the c1, c2 1e-6, c3, c4 and c5 entries")
     ap.add_argument("--cpu-iters", type=int, default=80, help="iteration limit of the CPU baseline sample (about 15 s)")
-    ap.add_argument("--instances", type=int, default=8, help="held-out instances of the gmm_warm_start workload")
+    ap.add_argument("--instances", type=int, default=8, help="held-out instances of the warm-start workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="gaussian_shift",
                     choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform", "gmm_warm_start", "precision"),
-                    help="gaussian_shift: the headline 10k x 10k measurement; the others measure the 30k x 30k "
-                         "FP32-storage stress case, the 512-instance batch, the learned warm start and (about "
-                         "15 minutes) FP32 storage against the FP64 1e-8 solve at 10k x 10k")
+                    help="gaussian_shift: the headline 10k x 10k measurement (plus the other workloads' entries); "
+                         "the others measure one workload alone: the 30k x 30k FP32-storage stress case (with its "
+                         "point-cloud form), the 512-instance batch, the learned warm start and (about 15 minutes) "
+                         "FP32 storage against the FP64 1e-8 solve at 10k x 10k")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -354,30 +530,19 @@ def main() -> int:
         t0 = time.time()
         dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
         up = time.time() - t0
-        ks, tt, st = measure_steps(dev, args.steps, args.warmup)
-        alg = algorithmic_bytes_per_step(n, m, elem)
-        k_avg = ks / args.steps * 1e-3
-        results[name] = {
-            "ms_per_step": tt / args.steps, "iters_per_s": args.steps / (tt * 1e-3),
-            "step_kernel_us": k_avg * 1e6, "step_kernel_GBs": alg / k_avg / 1e9,
-            "iteration_GBs": alg / (tt / args.steps * 1e-3) / 1e9, "bytes_per_step": alg,
-            "launch": dev.get_launch(), "numerical_error": int(st.numerical_error),
-            "upload_sec": up, "upload_GBs_pcie": 8 * n * m / up / 1e9,
-        }
+        results[name] = step_measurement(dev, n, m, elem, args.steps, args.warmup)
+        results[name].update({"iteration_GBs": results[name]["bytes_per_step"] / (results[name]["ms_per_step"] * 1e-3) / 1e9,
+                              "launch": dev.get_launch(), "upload_sec": up, "upload_GBs_pcie": 8 * n * m / up / 1e9})
         devs[name] = dev
 
-    # point-cloud cost form (C formed in registers, never stored): 4 n*m streams per step
+    # point-cloud cost form (C formed in registers, never stored): 3 n*m streams per step on average
     points = {}
     pts = instances.gaussian_shift_points(n, m)
     for storage, name, elem in ((_lib.STORAGE_F64, "f64", 8), (_lib.STORAGE_F32, "f32", 4)):
         dev = _lib.DeviceSolver(n, m, storage=storage)
         dev.upload_points(pts)
-        ks, tt, st = measure_steps(dev, args.steps, args.warmup)
-        alg = algorithmic_bytes_per_step(n, m, elem, matrix_streams=3)
-        k_avg = ks / args.steps * 1e-3
-        points[name] = {"ms_per_step": tt / args.steps, "iters_per_s": args.steps / (tt * 1e-3),
-                        "step_kernel_us": k_avg * 1e6, "step_kernel_GBs": alg / k_avg / 1e9, "bytes_per_step": alg,
-                        "device_GB": dev.device_bytes() / 1e9}
+        points[name] = step_measurement(dev, n, m, elem, args.steps, args.warmup, matrix_streams=3)
+        points[name]["device_GB"] = dev.device_bytes() / 1e9
         if not args.no_solve:
             points[name]["time_to_1e-4"] = solve_to_tol(pts, storage, 1e-4, dev=dev)
         dev.close()
@@ -413,7 +578,9 @@ def main() -> int:
     if not args.no_cpu:
         cpu = cpu_baseline(prob, args.cpu_iters)
 
-    f64 = results["f64"]
+    f64, f32 = results["f64"], results["f32"]
+    f32_roof = roofline(f32["step_kernel_GBs"], f32["bytes_per_step"], f32["step_kernel_us"],
+                        "step_kernel<float>", measured_traffic(n, m, "f32"))
     line = {
         "metric": METRIC,
         "value": f64["iters_per_s"],
@@ -429,18 +596,18 @@ def main() -> int:
         "data": "synthetic",
         "config": {"workload": f"gaussian_shift {n}x{m} FP64 (squared Euclidean / max, U(0,1) masses, seed 2000)",
                    "n": n, "m": m, "storage": "f64", "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": f64["step_kernel_GBs"], "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": f64["step_kernel_GBs"] / PEAK_HBM_GBS, "traffic": measured_traffic(n, m, "f64"),
-                     "kernel": "step_kernel<double>", "bytes_per_launch": f64["bytes_per_step"],
-                     "avg_launch_us": f64["step_kernel_us"]},
+        "roofline": roofline(f64["step_kernel_GBs"], f64["bytes_per_step"], f64["step_kernel_us"],
+                             "step_kernel<double>", measured_traffic(n, m, "f64")),
         "cpu_baseline": cpu,
         "iteration_GBs": f64["iteration_GBs"],
-        "f32_storage": results["f32"],
+        "f32_storage": dict(f32, roofline=f32_roof),
         "time_to_1e-4": solves or None,
         "point_cloud_cost": points,
     }
     if copy_probe:
         line["copy_probe"] = copy_probe
+    if not (args.no_solve or args.no_extra) and n == 10000:
+        line["workloads"] = extras(args, prob, f32_roof)
     print(json.dumps(line))
     return 0
 

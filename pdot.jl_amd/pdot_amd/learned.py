@@ -219,6 +219,11 @@ def load_initial_primal_weight(path: str = DEFAULT_WEIGHTS) -> Optional[float]:
     return float(meta["initial_primal_weight"]) if "initial_primal_weight" in meta else None
 
 
+def has_weight_head(path: str = DEFAULT_WEIGHTS) -> bool:
+    """Whether the weights file carries the per-instance initial-weight head."""
+    return "weight_head" in load_meta(path)
+
+
 def zero_start_relative_kkt(problem) -> float:
     """Relative KKT error of the zero start at its first evaluation, in closed
     form: X = 0, u = v = 0 (so G = 0) leave only the primal residual
