@@ -321,6 +321,17 @@ int pdot_mlp_forward(pdot_mlp* m, const float* features, int64_t nsrc, int64_t n
 int pdot_mlp_loss_grad(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, const float* target,
                        float* loss, float* grad);
 int pdot_mlp_adam_step(pdot_mlp* m, float lr, float beta1, float beta2, float eps);
+/* The weight head: the natural log of a per-instance initial primal weight
+ * (the reference's primal_importance, src/solver_gpu.jl:486-496) predicted
+ * from the mean-pooled context of the first layer and the instance size.
+ * nsrc, ntgt >= 1. */
+int pdot_mlp_predict_weight(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, float* log_weight);
+/* Squared error of the weight head against target (a log weight) and its
+ * gradient (num_params, may be NULL; zero outside the head, and outside the
+ * head and the first layer when train_trunk), kept on the device for
+ * pdot_mlp_adam_step; log_weight (may be NULL) receives the prediction. */
+int pdot_mlp_weight_loss_grad(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, float target,
+                              int32_t train_trunk, float* log_weight, float* loss, float* grad);
 /* Test hook for the MFMA matrix product: C (M x N) = op(A) op(B), operands
  * rounded to bf16, FP32 accumulation; row-major, op = transpose if set. */
 int pdot_mlp_gemm_probe(pdot_mlp* m, const float* A, const float* B, float* C, int64_t M, int64_t N, int64_t K,

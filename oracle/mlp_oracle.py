@@ -16,8 +16,11 @@ import numpy as np
 
 W = 128
 F = 4
+H = 32
 SHAPES = [("W1", (F, W)), ("b1", (W,)), ("W2a", (W, W)), ("W2b", (W, W)), ("W2c", (W, W)), ("b2", (W,)),
-          ("W3", (W, W)), ("b3", (W,)), ("w4", (W,)), ("b4", (1,))]
+          ("W3", (W, W)), ("b3", (W,)), ("w4", (W,)), ("b4", (1,)),
+          ("Wh1a", (W, H)), ("Wh1b", (W, H)), ("wh1s", (H,)), ("bh1", (H,)), ("wh2", (H,)), ("bh2", (1,))]
+HEAD = ("Wh1a", "Wh1b", "wh1s", "bh1", "wh2", "bh2")
 NUM_PARAMS = sum(int(np.prod(s)) for _, s in SHAPES)
 
 
@@ -93,7 +96,54 @@ def loss_grad(params, feats, ns: int, nt: int, target, emulate: bool = True):
     dZ1 = dH1 * (c["Z1"] > 0)
     g["W1"] = _mm(c["X0"].T, dZ1, emulate)
     g["b1"] = dZ1.sum(axis=0)
+    for name in HEAD:  # the dual loss does not reach the weight head
+        g[name] = np.zeros(dict(SHAPES)[name])
     return loss, pack(g), out
+
+
+def size_feature(ns: int, nt: int) -> float:
+    return (np.log2(max(ns, 1)) + np.log2(max(nt, 1))) / 24.0
+
+
+def forward_weight(params, feats, ns: int, nt: int, emulate: bool = True):
+    """The weight head: log of the initial primal weight.  The head itself is
+    plain arithmetic (no bf16 operands); its inputs come from layer 1."""
+    P = unpack(params)
+    X0 = np.asarray(feats, dtype=np.float64).reshape(ns + nt, F)
+    Z1 = _mm(X0, P["W1"], emulate) + P["b1"]
+    H1 = np.maximum(Z1, 0.0)
+    cs, ct = H1[:ns].mean(axis=0), H1[ns:].mean(axis=0)
+    s = size_feature(ns, nt)
+    zh = ((cs @ P["Wh1a"] + ct @ P["Wh1b"]) + s * P["wh1s"]) + P["bh1"]
+    hh = np.maximum(zh, 0.0)
+    out = float(hh @ P["wh2"] + P["bh2"][0])
+    return out, dict(P=P, X0=X0, Z1=Z1, cs=cs, ct=ct, s=s, zh=zh, hh=hh)
+
+
+def weight_loss_grad(params, feats, ns: int, nt: int, target: float, train_trunk: bool = False,
+                     emulate: bool = True):
+    """(log_weight - target)^2 and its gradient (zero outside the head, and
+    outside the head and layer 1 when train_trunk)."""
+    out, c = forward_weight(params, feats, ns, nt, emulate)
+    P = c["P"]
+    dl = 2.0 * (out - target)
+    g = {name: np.zeros(shape) for name, shape in SHAPES}
+    g["wh2"] = dl * c["hh"]
+    g["bh2"] = np.array([dl])
+    dz = dl * P["wh2"] * (c["zh"] > 0)
+    g["bh1"] = dz
+    g["wh1s"] = c["s"] * dz
+    g["Wh1a"] = np.outer(c["cs"], dz)
+    g["Wh1b"] = np.outer(c["ct"], dz)
+    if train_trunk:
+        dcs, dct = P["Wh1a"] @ dz, P["Wh1b"] @ dz
+        dH1 = np.zeros((ns + nt, W))
+        dH1[:ns] += dcs / ns
+        dH1[ns:] += dct / nt
+        dZ1 = dH1 * (c["Z1"] > 0)
+        g["W1"] = _mm(c["X0"].T, dZ1, emulate)
+        g["b1"] = dZ1.sum(axis=0)
+    return (out - target) ** 2, pack(g), out
 
 
 def gemm(A, B, trans_a=False, trans_b=False):

@@ -14,9 +14,19 @@
 //   Z2 = H1 W2a + g,          H2 = relu(Z2)
 //   Z3 = H2 W3 + b3,          H3 = relu(Z3)
 //   out = H3 w4 + b4                                           N x 1
-// Loss: mean((out - target)^2).  Parameters (flat FP32, this order):
+// Loss: mean((out - target)^2).
+//
+// Weight head (one scalar per instance, the log of the initial primal
+// weight), from the pooled context and the size s = (log2 ns + log2 nt) / 24:
+//   zh = cs Wh1a + ct Wh1b + s wh1s + bh1,  hh = relu(zh)       (1 x 32)
+//   log_weight = hh . wh2 + bh2
+// Loss: (log_weight - target)^2.  Its gradient reaches W1 and b1 through the
+// pooling only when the trunk is trained with it.
+//
+// Parameters (flat FP32, this order):
 //   W1 4x128, b1 128, W2a 128x128, W2b 128x128, W2c 128x128, b2 128,
-//   W3 128x128, b3 128, w4 128, b4 1.
+//   W3 128x128, b3 128, w4 128, b4 1,
+//   Wh1a 128x32, Wh1b 128x32, wh1s 32, bh1 32, wh2 32, bh2 1.
 
 #include <hip/hip_runtime.h>
 
@@ -35,6 +45,7 @@ constexpr int kF = 4;     // input features
 constexpr int kFP = 16;   // input features padded to the MFMA k-step
 constexpr int kT = 32;    // MFMA output tile (32 x 32)
 constexpr int kKs = 16;   // MFMA k-step
+constexpr int kH = 32;    // weight-head width
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -195,6 +206,69 @@ __global__ void pad_kernel(const float* src, int64_t rows, int cols, float* dst,
     dst[t] = c < cols ? src[i * cols + c] : 0.f;
 }
 
+// The instance-size input of the weight head.
+__device__ __forceinline__ float size_feature(int64_t ns, int64_t nt) {
+    return (log2f((float)(ns > 0 ? ns : 1)) + log2f((float)(nt > 0 ? nt : 1))) * (1.f / 24.f);
+}
+
+// Weight head forward, one wave: lanes c < 32 form zh_c (saved) and
+// hh_c = relu(zh_c); out[0] = sum_c hh_c wh2_c + bh2 (a shuffle tree over
+// the 32 lanes, fixed order).  Plain FP32: the head is 8 K multiply-adds.
+__global__ __launch_bounds__(64) void weight_head_kernel(const float* cs, const float* ct, const float* P,
+                                                         int64_t oWa, int64_t oWb, int64_t ows, int64_t obh,
+                                                         int64_t ow2, int64_t ob2, int64_t ns, int64_t nt, float* zh,
+                                                         float* out) {
+    const int c = threadIdx.x;
+    float h = 0.f;
+    if (c < kH) {
+        float sa = 0.f, sb = 0.f;
+        for (int k = 0; k < kW; ++k) sa += cs[k] * P[oWa + k * kH + c];
+        for (int k = 0; k < kW; ++k) sb += ct[k] * P[oWb + k * kH + c];
+        const float z = ((sa + sb) + size_feature(ns, nt) * P[ows + c]) + P[obh + c];
+        zh[c] = z;
+        h = (z > 0.f ? z : 0.f) * P[ow2 + c];
+    }
+    for (int off = 16; off >= 1; off >>= 1) h += __shfl_down(h, off, 64);
+    if (c == 0) out[0] = h + P[ob2];
+}
+
+// Weight head backward for the loss (log_weight - target)^2, one block of
+// kW threads: the head's gradients into G (their offsets), and, when dcs is
+// not null, the gradient reaching the pooled context (dcs, dct).
+__global__ __launch_bounds__(128) void weight_head_back_kernel(const float* cs, const float* ct, const float* P,
+                                                               float* G, int64_t oWa, int64_t oWb, int64_t ows,
+                                                               int64_t obh, int64_t ow2, int64_t ob2, int64_t ns,
+                                                               int64_t nt, const float* zh, const float* out,
+                                                               float target, float* dcs, float* dct) {
+    __shared__ float dz[kH];
+    const int t = threadIdx.x;
+    const float dl = 2.f * (out[0] - target);
+    if (t < kH) {
+        const float z = zh[t];
+        const float hh = z > 0.f ? z : 0.f;
+        G[ow2 + t] = dl * hh;
+        const float d = z > 0.f ? dl * P[ow2 + t] : 0.f;
+        dz[t] = d;
+        G[obh + t] = d;
+        G[ows + t] = size_feature(ns, nt) * d;
+    }
+    if (t == 0) G[ob2] = dl;
+    __syncthreads();
+    // row k of Wh1a / Wh1b
+    const int k = t;
+    float sa = 0.f, sb = 0.f;
+    for (int c = 0; c < kH; ++c) {
+        G[oWa + k * kH + c] = cs[k] * dz[c];
+        G[oWb + k * kH + c] = ct[k] * dz[c];
+        sa += P[oWa + k * kH + c] * dz[c];
+        sb += P[oWb + k * kH + c] * dz[c];
+    }
+    if (dcs) {
+        dcs[k] = sa;
+        dct[k] = sb;
+    }
+}
+
 unsigned blocks_for(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
 }  // namespace
@@ -204,6 +278,7 @@ struct pdot_mlp {
     hipStream_t stream = nullptr;
     // parameter offsets in the flat vector
     int64_t oW1, ob1, oW2a, oW2b, oW2c, ob2, oW3, ob3, ow4, ob4, np;
+    int64_t oWha, oWhb, owhs, obh1, owh2, obh2;  // weight head
     float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  // params, grads, Adam moments
     int64_t adam_t = 0;
     // activations for the current N
@@ -213,6 +288,7 @@ struct pdot_mlp {
     float *feat = nullptr;
     float *W1p = nullptr, *dW1p = nullptr, *w4p = nullptr;  // padded W1 (16x128), its gradient, w4 as 128 x 32
     float *cs = nullptr, *ct = nullptr, *gctx = nullptr, *dcs = nullptr, *dct = nullptr, *dg = nullptr;
+    float *zh = nullptr, *logw = nullptr;  // weight head pre-activations and output
     std::vector<float> hbuf;
     std::string err;
 };
@@ -266,8 +342,8 @@ hipError_t gemm(pdot_mlp* m, const float* A, int64_t lda, const float* B, int64_
     return hipGetLastError();
 }
 
-// Forward pass on the features already in m->feat; fills all activations.
-int forward(pdot_mlp* m, int64_t ns, int64_t nt, const float* target_dev) {
+// Layer 1 and the pooled context on the features already in m->feat.
+int forward_trunk(pdot_mlp* m, int64_t ns, int64_t nt) {
     const int64_t N = ns + nt;
     float* P = m->p;
     hipLaunchKernelGGL(pad_kernel, dim3(blocks_for(N * kFP)), dim3(256), 0, m->stream, m->feat, N, kF, m->X0, kFP);
@@ -278,6 +354,24 @@ int forward(pdot_mlp* m, int64_t ns, int64_t nt, const float* target_dev) {
     hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->Z1, P + m->ob1, m->Z1,
                        m->H1, N, kW);
     hipLaunchKernelGGL(pool_kernel, dim3(1), dim3(kW), 0, m->stream, m->H1, ns, nt, kW, m->cs, m->ct);
+    MHIPCHK(m, hipGetLastError());
+    return PDOT_OK;
+}
+
+// The weight head after forward_trunk: m->zh, m->logw.
+int forward_weight(pdot_mlp* m, int64_t ns, int64_t nt) {
+    hipLaunchKernelGGL(weight_head_kernel, dim3(1), dim3(64), 0, m->stream, m->cs, m->ct, m->p, m->oWha, m->oWhb,
+                       m->owhs, m->obh1, m->owh2, m->obh2, ns, nt, m->zh, m->logw);
+    MHIPCHK(m, hipGetLastError());
+    return PDOT_OK;
+}
+
+// Forward pass on the features already in m->feat; fills all activations.
+int forward(pdot_mlp* m, int64_t ns, int64_t nt, const float* target_dev) {
+    const int64_t N = ns + nt;
+    float* P = m->p;
+    int rc = forward_trunk(m, ns, nt);
+    if (rc) return rc;
     hipLaunchKernelGGL(context_kernel, dim3(1), dim3(kW), 0, m->stream, m->cs, m->ct, P + m->oW2b, P + m->oW2c,
                        P + m->ob2, m->gctx, kW);
     MHIPCHK(m, (gemm<false, false>(m, m->H1, kW, P + m->oW2a, kW, m->Z2, kW, N, kW, kW)));
@@ -302,6 +396,8 @@ int backward(pdot_mlp* m, int64_t ns, int64_t nt) {
     const int64_t N = ns + nt;
     float* P = m->p;
     float* G = m->g;
+    // the dual loss does not reach the weight head
+    MHIPCHK(m, hipMemsetAsync(G + m->oWha, 0, sizeof(float) * (m->np - m->oWha), m->stream));
     // head: dW4 = H3^T dout (128 x 1), db4 = sum dout, dH3 = dout w4^T
     MHIPCHK(m, (gemm<true, false>(m, m->H3, kW, m->dout, 1, G + m->ow4, 1, kW, 1, N)));
     hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1), 0, m->stream, m->dout, N, 1, G + m->ob4);
@@ -325,6 +421,29 @@ int backward(pdot_mlp* m, int64_t ns, int64_t nt) {
     MHIPCHK(m, (gemm<true, false>(m, m->X0, kFP, m->D, kW, m->dW1p, kW, kFP, kW, N)));
     MHIPCHK(m, hipMemcpyAsync(G + m->oW1, m->dW1p, sizeof(float) * kF * kW, hipMemcpyDeviceToDevice, m->stream));
     hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, G + m->ob1);
+    MHIPCHK(m, hipGetLastError());
+    return PDOT_OK;
+}
+
+// Gradient of (log_weight - target)^2 after forward_trunk + forward_weight:
+// the head's entries of m->g and, if train_trunk, W1 and b1 through the
+// pooling; every other entry is zero.
+int backward_weight(pdot_mlp* m, int64_t ns, int64_t nt, float target, bool train_trunk) {
+    const int64_t N = ns + nt;
+    MHIPCHK(m, hipMemsetAsync(m->g, 0, sizeof(float) * m->np, m->stream));
+    hipLaunchKernelGGL(weight_head_back_kernel, dim3(1), dim3(kW), 0, m->stream, m->cs, m->ct, m->p, m->g, m->oWha,
+                       m->oWhb, m->owhs, m->obh1, m->owh2, m->obh2, ns, nt, m->zh, m->logw, target,
+                       train_trunk ? m->dcs : nullptr, train_trunk ? m->dct : nullptr);
+    if (train_trunk) {
+        MHIPCHK(m, hipMemsetAsync(m->D, 0, sizeof(float) * N * kW, m->stream));
+        hipLaunchKernelGGL(pool_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->D, m->dcs, m->dct,
+                           ns, nt, kW);
+        hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->D, m->Z1, N * kW);
+        MHIPCHK(m, (gemm<true, false>(m, m->X0, kFP, m->D, kW, m->dW1p, kW, kFP, kW, N)));
+        MHIPCHK(m, hipMemcpyAsync(m->g + m->oW1, m->dW1p, sizeof(float) * kF * kW, hipMemcpyDeviceToDevice,
+                                  m->stream));
+        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, m->g + m->ob1);
+    }
     MHIPCHK(m, hipGetLastError());
     return PDOT_OK;
 }
@@ -359,6 +478,12 @@ int pdot_mlp_create(pdot_mlp** out, int32_t device) {
     m->ob3 = o; o += kW;
     m->ow4 = o; o += kW;
     m->ob4 = o; o += 1;
+    m->oWha = o; o += kW * kH;
+    m->oWhb = o; o += kW * kH;
+    m->owhs = o; o += kH;
+    m->obh1 = o; o += kH;
+    m->owh2 = o; o += kH;
+    m->obh2 = o; o += 1;
     m->np = o;
     int rc;
     float** pv[] = {&m->p, &m->g, &m->m, &m->v};
@@ -369,7 +494,7 @@ int pdot_mlp_create(pdot_mlp** out, int32_t device) {
     if ((rc = malloc_f(m, &m->W1p, kFP * kW))) return rc;
     if ((rc = malloc_f(m, &m->dW1p, kFP * kW))) return rc;
     if ((rc = malloc_f(m, &m->w4p, kW * kT))) return rc;
-    float** sm[] = {&m->cs, &m->ct, &m->gctx, &m->dcs, &m->dct, &m->dg};
+    float** sm[] = {&m->cs, &m->ct, &m->gctx, &m->dcs, &m->dct, &m->dg, &m->zh, &m->logw};
     for (float** q : sm)
         if ((rc = malloc_f(m, q, kW))) return rc;
     MHIPCHK(m, hipStreamSynchronize(m->stream));
@@ -381,7 +506,7 @@ void pdot_mlp_destroy(pdot_mlp* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     float* all[] = {m->p, m->g, m->m, m->v, m->X0, m->Z1, m->H1, m->Z2, m->H2, m->Z3, m->H3, m->Y4, m->out, m->tgt,
                     m->dout, m->sq, m->D, m->E, m->feat, m->W1p, m->dW1p, m->w4p, m->cs, m->ct, m->gctx, m->dcs,
-                    m->dct, m->dg};
+                    m->dct, m->dg, m->zh, m->logw};
     for (float* q : all)
         if (q) (void)hipFree(q);
     if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -439,6 +564,34 @@ int pdot_mlp_loss_grad(pdot_mlp* m, const float* features, int64_t nsrc, int64_t
         for (int64_t i = 0; i < N; ++i) s += m->hbuf[i];
         *loss = (float)(s / (double)N);
     }
+    return PDOT_OK;
+}
+
+int pdot_mlp_predict_weight(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, float* log_weight) {
+    if (!m || !features || !log_weight || nsrc < 1 || ntgt < 1) return PDOT_E_ARG;
+    int rc = upload_inputs(m, features, nsrc + ntgt, nullptr);
+    if (rc) return rc;
+    if ((rc = forward_trunk(m, nsrc, ntgt))) return rc;
+    if ((rc = forward_weight(m, nsrc, ntgt))) return rc;
+    MHIPCHK(m, hipMemcpyAsync(log_weight, m->logw, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    return PDOT_OK;
+}
+
+int pdot_mlp_weight_loss_grad(pdot_mlp* m, const float* features, int64_t nsrc, int64_t ntgt, float target,
+                              int32_t train_trunk, float* log_weight, float* loss, float* grad) {
+    if (!m || !features || nsrc < 1 || ntgt < 1 || !(target == target)) return PDOT_E_ARG;
+    int rc = upload_inputs(m, features, nsrc + ntgt, nullptr);
+    if (rc) return rc;
+    if ((rc = forward_trunk(m, nsrc, ntgt))) return rc;
+    if ((rc = forward_weight(m, nsrc, ntgt))) return rc;
+    if ((rc = backward_weight(m, nsrc, ntgt, target, train_trunk != 0))) return rc;
+    float lw = 0.f;
+    MHIPCHK(m, hipMemcpyAsync(&lw, m->logw, sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    if (grad) MHIPCHK(m, hipMemcpyAsync(grad, m->g, sizeof(float) * m->np, hipMemcpyDeviceToHost, m->stream));
+    MHIPCHK(m, hipStreamSynchronize(m->stream));
+    if (log_weight) *log_weight = lw;
+    if (loss) *loss = (lw - target) * (lw - target);
     return PDOT_OK;
 }
 

@@ -153,6 +153,8 @@ SIGNATURES = {
     "pdot_mlp_forward": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, ctypes.c_int64, _fp]),
     "pdot_mlp_loss_grad": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, ctypes.c_int64, _fp, _fp, _fp]),
     "pdot_mlp_adam_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "pdot_mlp_predict_weight": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, ctypes.c_int64, _fp]),
+    "pdot_mlp_weight_loss_grad": (ctypes.c_int, [ctypes.c_void_p, _fp, ctypes.c_int64, ctypes.c_int64, ctypes.c_float, ctypes.c_int32, _fp, _fp, _fp]),
     "pdot_mlp_gemm_probe": (ctypes.c_int, [ctypes.c_void_p, _fp, _fp, _fp, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32]),
 }
 

@@ -60,8 +60,25 @@ def optimize_batch(params: PdhgParameters, problems: Sequence[OptimalTransportPr
                    profile: Optional[BatchProfile] = None) -> List[SaddlePointOutput]:
     """optimize() for every problem, advanced together.  warm_starts: optional
     list of (u0, v0) or None per problem.  batch: reuse a BatchSolver whose
-    problems are already uploaded."""
+    problems are already uploaded.  params may also be a list with one entry
+    per problem (for example per-instance initial primal weights); the entries
+    must share the evaluation frequency, and their iteration limits must equal
+    the first entry's or be multiples of the frequency, so that every
+    instance's evaluations fall on the common schedule."""
     count = len(problems)
+    plist = list(params) if isinstance(params, (list, tuple)) else [params] * count
+    if len(plist) != count:
+        raise ValueError("one PdhgParameters per problem expected")
+    params = plist[0]
+    freq = params.termination_evaluation_frequency
+    for p in plist:
+        lim = p.termination_criteria.iteration_limit
+        if p.termination_evaluation_frequency != freq:
+            raise ValueError("every instance needs the same termination_evaluation_frequency")
+        if p.step_size_policy_params != params.step_size_policy_params:
+            raise ValueError("every instance needs the same step-size policy")
+        if lim != params.termination_criteria.iteration_limit and not (lim > 10 and lim % freq == 0):
+            raise ValueError("per-instance iteration limits must be multiples of the evaluation frequency")
     caches = [cached_ot_info(p) for p in problems]
     t_up = time.time()
     if batch is None:
@@ -71,19 +88,18 @@ def optimize_batch(params: PdhgParameters, problems: Sequence[OptimalTransportPr
             batch.upload(k, p.cost_matrix, p.source_distribution, p.target_distribution)
     upload_sec = time.time() - t_up
     batch.use_graphs(use_graphs)
-    init = [initial_step_and_weight(params, p.num_row, p.num_col, caches[k]) for k, p in enumerate(problems)]
+    init = [initial_step_and_weight(plist[k], p.num_row, p.num_col, caches[k]) for k, p in enumerate(problems)]
     policy, re, ge = init[0][0], init[0][1], init[0][2]
     batch.set_start(policy, re, ge, [x[3] for x in init], [x[4] for x in init])
     if warm_starts is not None:
         for k, ws in enumerate(warm_starts):
             if ws is not None:
                 batch.set_warm_start(k, ws[0], ws[1])
-    ctls = [InstanceController(params, caches[k]) for k in range(count)]
+    ctls = [InstanceController(plist[k], caches[k]) for k in range(count)]
     outputs: List[Optional[SaddlePointOutput]] = [None] * count
     finish = [0.0] * count
     active = np.ones(count, dtype=np.int32)
     iteration_limit = params.termination_criteria.iteration_limit
-    freq = params.termination_evaluation_frequency
     start_time = time.time()
     time_basic = 0.0
     passes = 0

@@ -160,13 +160,20 @@ class PointCloudInstance:
     problem: OptimalTransportProblem
 
 
-def gmm_dirichlet(n: int = 4096, m: Optional[int] = None, seed: int = 5000) -> PointCloudInstance:
+def gmm_dirichlet_points(n: int = 4096, m: Optional[int] = None, seed: int = 5000):
+    """The point clouds and masses (x, y, a, b) of gmm_dirichlet(n, m, seed),
+    without forming the cost matrix."""
     m = n if m is None else m
     g = Rng(seed)
     x = _gmm_points(g, n)
     y = _gmm_points(g, m)
     a = g.dirichlet1(n)
     b = g.dirichlet1(m)
+    return x, y, a, b
+
+
+def gmm_dirichlet(n: int = 4096, m: Optional[int] = None, seed: int = 5000) -> PointCloudInstance:
+    x, y, a, b = gmm_dirichlet_points(n, m, seed)
     return PointCloudInstance(x, y, OptimalTransportProblem(sqeuclid_cost(x, y), a, b))
 
 

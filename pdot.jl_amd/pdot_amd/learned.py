@@ -5,10 +5,13 @@ from solves of the problem family.
   dual potentials (u, v) of a transport problem from its point clouds and
   masses; they set the starting iterate (X0 = 0, G0 = (C - u0) - v0).
 * Primal weight: the solver state's initial primal weight (the reference's
-  `primal_importance`, src/solver_gpu.jl:486-496, 1.0 by default), set to the
-  value that minimised the total iteration count over calibration solves of
-  the family (train_warm_start.calibrate_primal_weight), stored with the
-  network weights.
+  `primal_importance`, src/solver_gpu.jl:486-496, 1.0 by default).  The
+  calibrated constant is the grid weight with the fewest total iterations over
+  zero-start grid solves of training instances of the family
+  (train_warm_start).  The network's weight head predicts a per-instance
+  weight from the pooled context and the instance size, trained on each
+  instance's best grid weight; it replaces the constant only if training
+  found that it lowers the validation instances' iterations.
 
 Zero-start PDHG with the caller's own parameters stays the fallback and the
 solver's own KKT error decides: predicted duals whose first evaluation is not
@@ -40,7 +43,10 @@ _fp = ctypes.POINTER(ctypes.c_float)
 MLP_SIGNATURES = {k: v for k, v in _lib.SIGNATURES.items() if k.startswith("pdot_mlp_")}
 
 HIDDEN = 128
+HEAD_WIDTH = 32
 NUM_FEATURES = 4
+# parameters before the weight head (a weights file without the head holds only these)
+NUM_DUAL_PARAMS = 4 * 128 + 128 + 3 * 128 * 128 + 128 + 128 * 128 + 128 + 128 + 1
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warm_start_weights.npz")
 
 
@@ -75,6 +81,17 @@ def init_params(seed: int = 0) -> np.ndarray:
     W = HIDDEN
     parts = [he(NUM_FEATURES, (NUM_FEATURES, W)), np.zeros(W), he(3 * W, (W, W)), he(3 * W, (W, W)),
              he(3 * W, (W, W)), np.zeros(W), he(W, (W, W)), np.zeros(W), he(W, (W,)) * 0.1, np.zeros(1)]
+    return np.concatenate([p.ravel() for p in parts] + [init_head(seed)]).astype(np.float32)
+
+
+def init_head(seed: int = 0) -> np.ndarray:
+    """Weight-head parameters (Wh1a, Wh1b, wh1s, bh1, wh2, bh2): He-uniform
+    first layer, small output layer, zero biases."""
+    rng = np.random.default_rng(10_000 + seed)
+    W, H = HIDDEN, HEAD_WIDTH
+    lim = np.sqrt(6.0 / (2 * W + 1))
+    parts = [rng.uniform(-lim, lim, (W, H)), rng.uniform(-lim, lim, (W, H)), rng.uniform(-lim, lim, H),
+             np.zeros(H), rng.uniform(-0.1, 0.1, H), np.zeros(1)]
     return np.concatenate([p.ravel() for p in parts]).astype(np.float32)
 
 
@@ -157,6 +174,29 @@ class WarmStartNet:
         out = self.forward(features(x, y, a, b), n, m).astype(np.float64)
         return out[:n], out[n:]
 
+    def log_weight(self, feats, ns: int, nt: int) -> float:
+        lw = ctypes.c_float()
+        f, fp = _f32(feats)
+        self._check(self.lib.pdot_mlp_predict_weight(self.h, fp, int(ns), int(nt), ctypes.byref(lw)),
+                    "pdot_mlp_predict_weight")
+        return float(lw.value)
+
+    def predict_weight(self, x, y, a, b) -> float:
+        """The per-instance initial primal weight of the weight head."""
+        return float(np.exp(self.log_weight(features(x, y, a, b), len(x), len(y))))
+
+    def weight_loss_grad(self, feats, ns: int, nt: int, target: float, train_trunk: bool = False,
+                         want_grad: bool = False):
+        """Squared error of the log weight; returns (loss, prediction, gradient or None)."""
+        f, fp = _f32(feats)
+        lw, loss = ctypes.c_float(), ctypes.c_float()
+        g = np.empty(self.num_params, dtype=np.float32) if want_grad else None
+        self._check(self.lib.pdot_mlp_weight_loss_grad(self.h, fp, int(ns), int(nt), float(target), int(train_trunk),
+                                                       ctypes.byref(lw), ctypes.byref(loss),
+                                                       g.ctypes.data_as(_fp) if g is not None else None),
+                    "pdot_mlp_weight_loss_grad")
+        return float(loss.value), float(lw.value), g
+
 
 @dataclass
 class Sample:
@@ -202,8 +242,13 @@ def save_params(path: str, params: np.ndarray, meta: dict) -> None:
 
 
 def load_params(path: str = DEFAULT_WEIGHTS) -> np.ndarray:
+    """The network parameters; a file written before the weight head existed
+    gets a freshly initialised head (has_weight_head() is then False)."""
     with np.load(path, allow_pickle=False) as z:
-        return z["params"].astype(np.float32)
+        p = z["params"].astype(np.float32)
+    if p.size == NUM_DUAL_PARAMS:
+        p = np.concatenate([p, init_head()])
+    return p
 
 
 def load_meta(path: str = DEFAULT_WEIGHTS) -> dict:
@@ -220,8 +265,15 @@ def load_initial_primal_weight(path: str = DEFAULT_WEIGHTS) -> Optional[float]:
 
 
 def has_weight_head(path: str = DEFAULT_WEIGHTS) -> bool:
-    """Whether the weights file carries the per-instance initial-weight head."""
+    """Whether the weights file carries a trained per-instance weight head."""
     return "weight_head" in load_meta(path)
+
+
+def weight_head_selected(path: str = DEFAULT_WEIGHTS) -> bool:
+    """Whether training selected the head over the calibrated constant (it
+    lowered the validation instances' iterations)."""
+    meta = load_meta(path)
+    return bool(int(meta["weight_head_selected"])) if "weight_head_selected" in meta else False
 
 
 def zero_start_relative_kkt(problem) -> float:
@@ -242,6 +294,15 @@ class WarmStartInfo:
     predict_sec: float
 
 
+def learned_weight(net: WarmStartNet, x, y, a, b, path: str = DEFAULT_WEIGHTS) -> Optional[float]:
+    """The learned initial primal weight: the weight head's prediction when
+    training selected the head, else the calibrated constant (None if neither
+    exists)."""
+    if weight_head_selected(path):
+        return net.predict_weight(x, y, a, b)
+    return load_initial_primal_weight(path)
+
+
 def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True,
                primal_weight: Optional[float] = None, use_calibrated_weight: bool = True):
     """optimize() from the learned starting state, with the plain zero-start
@@ -250,8 +311,10 @@ def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE
     1. the predicted duals are evaluated (the first evaluation of a solve
        limited to one iteration); if their relative KKT error is not below the
        zero start's, the solve starts from zero duals;
-    2. the solve starts with the calibrated initial primal weight (or
-       `primal_weight`; use_calibrated_weight=False keeps the caller's);
+    2. the solve starts with the learned initial primal weight (the weight
+       head's per-instance prediction if training selected it, else the
+       calibrated constant; `primal_weight` overrides it and
+       use_calibrated_weight=False keeps the caller's);
     3. a warm solve that does not reach OPTIMAL is replaced by the plain
        zero-start solve with the caller's parameters.
 
@@ -261,9 +324,9 @@ def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE
     from .solve_log import TerminationReason
     t0 = time.time()
     u0, v0 = net.predict_duals(x, y, problem.source_distribution, problem.target_distribution)
-    pred = time.time() - t0
     if primal_weight is None and use_calibrated_weight:
-        primal_weight = load_initial_primal_weight()
+        primal_weight = learned_weight(net, x, y, problem.source_distribution, problem.target_distribution)
+    pred = time.time() - t0
     warm_params = params if primal_weight is None else replace(params, primal_importance=float(primal_weight))
     n, m = problem.cost_matrix.shape
     dev = _lib.DeviceSolver(n, m, storage=storage)

@@ -72,7 +72,7 @@ def fixture_batch():
 @pytest.mark.parametrize("storage,tol", [(_lib.STORAGE_F32, 1e-4), (_lib.STORAGE_F64, 1e-6)])
 def test_batch_results_match_lp_optima(storage, tol):
     cases, probs = fixture_batch()
-    outs = optimize_batch(P.default_parameters(tol), probs, storage=storage)
+    outs = optimize_batch(P.default_parameters(tol, iteration_limit=400000), probs, storage=storage)
     for c, prob, out in zip(cases, probs, outs):
         assert out.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL, c
         ci = out.iteration_stats[-1].convergence_information[0]
@@ -88,7 +88,7 @@ def test_batch_graph_replays_match_eager_bitwise():
     (with evaluations, restarts and host synchronisations between replays)
     gives bit-identical results to plain stream launches."""
     probs = mixed_problems()[:5]
-    params = P.default_parameters(1e-4)
+    params = P.default_parameters(1e-4, iteration_limit=400000)
     eager = optimize_batch(params, probs, storage=_lib.STORAGE_F32, use_graphs=False)
     graph = optimize_batch(params, probs, storage=_lib.STORAGE_F32, use_graphs=True)
     for e, g in zip(eager, graph):
@@ -102,7 +102,7 @@ def test_batch_graph_replays_match_eager_bitwise():
 
 def test_single_graph_replays_match_eager_bitwise():
     prob = instances.uniform_sqeuclid(400, 333, seed=4)
-    params = P.default_parameters(1e-4)
+    params = P.default_parameters(1e-4, iteration_limit=400000)
     e = P.optimize(params, prob, use_graphs=False)
     g = P.optimize(params, prob, use_graphs=True)
     assert e.iteration_count == g.iteration_count
@@ -111,7 +111,7 @@ def test_single_graph_replays_match_eager_bitwise():
 
 def test_batch_matches_single_instance_results():
     probs = mixed_problems()
-    params = P.default_parameters(1e-4)
+    params = P.default_parameters(1e-4, iteration_limit=400000)
     outs = optimize_batch(params, probs, storage=_lib.STORAGE_F64)
     for prob, out in zip(probs, outs):
         single = P.optimize(params, prob)
@@ -145,6 +145,6 @@ def test_autotune_applies_the_fastest_shape():
     res = autotune(b, steps=3)
     assert res["best"]["ms_per_step"] == min(t["ms_per_step"] for t in res["trials"])
     assert b.get_launch()["cols_per_tile"] == res["best"]["cols_per_tile"]
-    outs = optimize_batch(P.default_parameters(1e-4), probs, batch=b)
+    outs = optimize_batch(P.default_parameters(1e-4, iteration_limit=400000), probs, batch=b)
     assert all(o.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL for o in outs)
     b.close()

@@ -57,7 +57,7 @@ def test_deferred_solve_equals_eager_bitwise(form, storage, make):
     outs = []
     for defer in (True, False):
         dev = _device(prob, storage, defer, form == "points")
-        out = P.optimize(P.default_parameters(1e-4), prob, storage=storage, solver=dev)
+        out = P.optimize(P.default_parameters(1e-4, iteration_limit=400000), prob, storage=storage, solver=dev)
         avg = dev.download(_lib.POINT_AVERAGE)
         cur = dev.download(_lib.POINT_CURRENT)
         Y = dev.round(_lib.POINT_AVERAGE)
@@ -161,7 +161,7 @@ def test_batch_deferred_solve_equals_eager_bitwise(storage):
     runs = []
     for defer in (True, False):
         b = _batch(probs, storage, defer)
-        outs = optimize_batch(P.default_parameters(1e-4), probs, storage=storage, batch=b)
+        outs = optimize_batch(P.default_parameters(1e-4, iteration_limit=400000), probs, storage=storage, batch=b)
         states = b.get_states()
         avgs = [b.download(k, _lib.POINT_AVERAGE) if states[k].avg_count > 0 else None for k in range(len(probs))]
         b.close()

@@ -99,3 +99,65 @@ def test_training_reduces_loss(net):
     learned.train(net, samples, epochs=30, lr=3e-3)
     after = learned.evaluate_loss(net, samples)
     assert after < 0.2 * before
+
+
+@pytest.mark.parametrize("ns,nt,train_trunk", [(23, 17, False), (300, 257, True), (1, 64, True), (700, 3, False)])
+def test_weight_head_matches_oracle(net, ns, nt, train_trunk):
+    """The weight head's prediction, loss and gradient (FP32 head on the
+    bf16-MFMA first layer) against the restatement with the same roundings."""
+    rng = np.random.default_rng(ns * 13 + nt)
+    params = learned.init_params(seed=ns + 1)
+    params[np.arange(params.size) % 5 == 0] += 0.05
+    net.set_params(params)
+    feats = rng.random((ns + nt, 4)).astype(np.float32)
+    lw = net.log_weight(feats, ns, nt)
+    ref, _ = M.forward_weight(params.astype(np.float64), feats, ns, nt, emulate=True)
+    assert lw == pytest.approx(ref, rel=1e-4, abs=1e-5)
+    loss, pred, grad = net.weight_loss_grad(feats, ns, nt, -0.4, train_trunk, want_grad=True)
+    rl, rg, rout = M.weight_loss_grad(params.astype(np.float64), feats, ns, nt, -0.4, train_trunk, emulate=True)
+    assert pred == pytest.approx(rout, rel=1e-4, abs=1e-5)
+    assert loss == pytest.approx(rl, rel=1e-3, abs=1e-6)
+    o = 0
+    for name, shape in M.SHAPES:
+        k = int(np.prod(shape))
+        g, r = grad[o:o + k], rg[o:o + k]
+        if not np.any(r):
+            assert not np.any(g), name
+        else:
+            assert np.linalg.norm(g - r) <= 2e-3 * np.linalg.norm(r), name
+        o += k
+
+
+def test_dual_loss_leaves_weight_head_alone(net):
+    """After a weight-loss gradient, the dual loss's gradient is zero on the
+    head (no stale head gradient reaches Adam)."""
+    rng = np.random.default_rng(5)
+    net.set_params(learned.init_params(seed=5))
+    feats = rng.random((90, 4)).astype(np.float32)
+    net.weight_loss_grad(feats, 40, 50, 1.0, True)
+    _, g = net.loss_grad(feats, 40, 50, rng.standard_normal(90).astype(np.float32), want_grad=True)
+    assert not np.any(g[learned.NUM_DUAL_PARAMS:])
+    assert np.any(g[:learned.NUM_DUAL_PARAMS])
+
+
+def test_weight_head_training_fits_targets(net):
+    """Adam on the weight loss (head only) fits log weights that depend on
+    the point clouds' spread; the dual head's parameters do not move."""
+    rng = np.random.default_rng(21)
+    p0 = learned.init_params(seed=21)
+    net.set_params(p0)
+    data = []
+    for k in range(12):
+        spread = 0.2 + 0.8 * k / 11
+        x, y = spread * rng.random((80, 2)), spread * rng.random((70, 2))
+        a, b = rng.dirichlet(np.ones(80)), rng.dirichlet(np.ones(70))
+        data.append((learned.features(x, y, a, b), 80, 70, float(np.log(0.05 + 0.5 * spread))))
+    before = np.mean([net.weight_loss_grad(f, ns, nt, t)[0] for f, ns, nt, t in data])
+    for ep in range(150):
+        for f, ns, nt, t in data:
+            net.weight_loss_grad(f, ns, nt, t, False)
+            net.adam_step(3e-3)
+    after = np.mean([net.weight_loss_grad(f, ns, nt, t)[0] for f, ns, nt, t in data])
+    assert after < 0.1 * before
+    p1 = net.get_params()
+    assert np.array_equal(p1[:learned.NUM_DUAL_PARAMS], p0[:learned.NUM_DUAL_PARAMS])

@@ -114,3 +114,32 @@ def test_bench_byte_accounting_matches_the_traffic_profiles():
     assert bench.measured_traffic(5000, 5000, "f64") is None
     bt = bench.measured_batch_traffic()
     assert 8.8e9 < bt < 9.2e9
+
+
+def test_best_log_weight_parabola_and_edges():
+    """The weight target: the grid minimum of the iteration count, refined by
+    the parabola through log(iterations) around it (vertex of an exact
+    parabola recovered), kept at the minimum on the grid's edge or next to a
+    capped run, NaN when no run reached the tolerance."""
+    import math
+    from pdot_amd import train_warm_start as T
+    grid = (0.2, 0.1, 0.4, 0.05, 0.8)  # run order, not sorted
+    lw = np.log(np.array(grid))
+    vertex = math.log(0.15)
+    it = np.exp(2.0 * (lw - vertex) ** 2 + 7.0)
+    assert T.best_log_weight(grid, it) == pytest.approx(vertex, abs=1e-9)
+    # minimum at the edge of the grid
+    it2 = np.array([500, 400, 600, 300, 700])
+    assert T.best_log_weight(grid, it2) == pytest.approx(math.log(0.05))
+    # a capped neighbour (-1) keeps the grid minimum
+    it3 = np.array([300, 200, 500, -1, 700])
+    assert T.best_log_weight(grid, it3) == pytest.approx(math.log(0.1))
+    assert math.isnan(T.best_log_weight(grid, np.array([-1, -1, -1, -1, -1])))
+
+
+def test_calibrated_constant_minimises_total_iterations():
+    from pdot_amd import train_warm_start as T
+    data = {"grid": np.array([0.2, 0.1, 0.4]),
+            "iterations": np.array([[100, 300, 90], [200, 150, -1], [400, 100, 500]])}
+    # capped runs count as 3x the instance's best: totals 700, 550, 90 + 450 + 500 = 1040
+    assert T.calibrated_constant(data) == 0.1

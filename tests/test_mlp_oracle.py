@@ -55,3 +55,37 @@ def test_emulated_numerics_stay_close_to_fp64():
     lbf, gbf, _ = M.loss_grad(params, feats, ns, nt, target, emulate=True)
     assert lbf == pytest.approx(l64, rel=2e-2)
     assert np.linalg.norm(gbf - g64) / np.linalg.norm(g64) < 5e-2
+
+
+@pytest.mark.parametrize("seed,train_trunk", [(0, False), (1, True), (2, True)])
+def test_weight_head_gradient_matches_finite_differences(seed, train_trunk):
+    """The weight head's loss (log_weight - target)^2: its restated gradient
+    against central differences of the plain FP64 network, for the head's
+    blocks and, when the trunk is trained, W1 and b1; every other block is
+    zero."""
+    params, feats, _, ns, nt = _problem(seed)
+    target = 0.7
+    loss, grad, out = M.weight_loss_grad(params, feats, ns, nt, target, train_trunk, emulate=False)
+    assert loss == pytest.approx((out - target) ** 2)
+    rng = np.random.default_rng(200 + seed)
+    h = 1e-6
+    o = 0
+    for name, shape in M.SHAPES:
+        k = int(np.prod(shape))
+        live = name in M.HEAD or (train_trunk and name in ("W1", "b1"))
+        if not live:
+            assert not np.any(grad[o:o + k]), name
+        else:
+            for i in o + rng.choice(k, size=min(k, 6), replace=False):
+                p1, p2 = params.copy(), params.copy()
+                p1[i] += h
+                p2[i] -= h
+                fd = (M.weight_loss_grad(p1, feats, ns, nt, target, train_trunk, emulate=False)[0]
+                      - M.weight_loss_grad(p2, feats, ns, nt, target, train_trunk, emulate=False)[0]) / (2 * h)
+                assert grad[i] == pytest.approx(fd, rel=1e-5, abs=1e-8), (name, i)
+        o += k
+
+
+def test_weight_head_size_feature():
+    assert M.size_feature(4096, 4096) == 1.0
+    assert M.size_feature(1024, 2048) == pytest.approx(21 / 24)
