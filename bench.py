@@ -431,6 +431,53 @@ def bench_warm_start(args) -> int:
     return 0
 
 
+def bench_mlp(args) -> int:
+    """The warm-start network's kernels at the c5 scale (4096 source + 4096
+    target points): forward passes (dual prediction), training steps (loss,
+    gradient and Adam) and weight-head predictions, timed on the host around
+    the synchronous calls.  The MFMA products' rates come from the kernel
+    trace (rocprofv3) of the same command; this line reports the algorithmic
+    flops per call."""
+    import numpy as np
+    from pdot_amd import instances, learned
+    inst = instances.gmm_dirichlet(4096, seed=900200)
+    p = inst.problem
+    feats = learned.features(inst.x, inst.y, p.source_distribution, p.target_distribution)
+    ns = nt = 4096
+    N = ns + nt
+    net = learned.WarmStartNet(learned.load_params())
+    target = np.zeros(N, dtype=np.float32)
+    for _ in range(max(1, args.warmup)):
+        net.forward(feats, ns, nt)
+        net.loss_grad(feats, ns, nt, target)
+        net.adam_step(0.0)
+    t0 = time.time()
+    for _ in range(args.steps):
+        net.forward(feats, ns, nt)
+    fwd = (time.time() - t0) / args.steps
+    t0 = time.time()
+    for _ in range(args.steps):
+        net.loss_grad(feats, ns, nt, target)
+        net.adam_step(0.0)
+    step = (time.time() - t0) / args.steps
+    t0 = time.time()
+    for _ in range(args.steps):
+        net.log_weight(feats, ns, nt)
+    head = (time.time() - t0) / args.steps
+    net.close()
+    W, F = learned.HIDDEN, 16  # the first layer's inputs padded to the MFMA k-step
+    fwd_flops = 2 * N * (F * W + 2 * W * W + W * 32)  # layer 1, layers 2 and 3, the output (32-column tile)
+    print(json.dumps({"metric": "warm-start network at 4096 + 4096 points: forward and training step",
+                      "unit": "ms", "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
+                      "config": {"workload": "gmm_dirichlet 4096x4096 seed 900200 features", "points": N,
+                                 "hidden": W},
+                      "value": step * 1e3, "forward_ms": fwd * 1e3, "train_step_ms": step * 1e3,
+                      "weight_head_ms": head * 1e3, "forward_mfma_flops": fwd_flops,
+                      "largest_product_flops": 2 * N * W * W,
+                      "forward_TFLOPs": fwd_flops / fwd / 1e12}))
+    return 0
+
+
 def bench_precision(args) -> int:
     """FP32 storage against the FP64 oracle at 10000 x 10000 (gaussian shift):
     the FP64 solve tightened to 1e-8 and the FP32-storage solve to 1.5e-8,
@@ -500,11 +547,11 @@ def main() -> int:
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this many MiB (counter calibration)")
     ap.add_argument("--workload", default="gaussian_shift",
-                    choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform", "gmm_warm_start", "precision"),
+                    choices=("gaussian_shift", "l1_mixture_3d", "batched_uniform", "gmm_warm_start", "precision", "mlp"),
                     help="gaussian_shift: the headline 10k x 10k measurement (plus the other workloads' entries); "
                          "the others measure one workload alone: the 30k x 30k FP32-storage stress case (with its "
-                         "point-cloud form), the 512-instance batch, the learned warm start and (about 15 minutes) "
-                         "FP32 storage against the FP64 1e-8 solve at 10k x 10k")
+                         "point-cloud form), the 512-instance batch, the learned warm start, the warm-start network's "
+                         "kernels and (about 15 minutes) FP32 storage against the FP64 1e-8 solve at 10k x 10k")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the library has no multi-GPU path (asked for --gpus {args.gpus})",
@@ -518,6 +565,8 @@ def main() -> int:
         return bench_precision(args)
     if args.workload == "gmm_warm_start":
         return bench_warm_start(args)
+    if args.workload == "mlp":
+        return bench_mlp(args)
 
     from pdot_amd import _lib, instances
 

@@ -30,6 +30,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -67,24 +68,32 @@ __device__ __forceinline__ __bf16 to_bf16(float f) {
 // output tile, k in steps of 16.  MFMA 32x32x16 operand layout: lane l holds
 // row (l % 32) of A and column (l % 32) of B, k = 8*(l / 32) + 0..7; the
 // 16 accumulators of lane l hold column (l % 32), rows 8*(r/4) + 4*(l/32) + r%4.
+// Split K: blockIdx.z takes k in [z*KS, min(K, (z+1)*KS)) and writes its
+// partial product to C + z*slice (summed by splitk_sum_kernel); the products
+// whose K is the point count (the weight gradients, M, N <= 128) would
+// otherwise leave 16 waves each walking thousands of k-steps.
 template <bool TA, bool TB>
 __global__ __launch_bounds__(64) void gemm_bf16_kernel(const float* __restrict__ A, int64_t lda,
                                                        const float* __restrict__ B, int64_t ldb, float* __restrict__ C,
-                                                       int64_t ldc, int64_t M, int64_t N, int64_t K) {
+                                                       int64_t ldc, int64_t M, int64_t N, int64_t K, int64_t KS,
+                                                       int64_t slice) {
     const int lane = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.x * kT, c0 = (int64_t)blockIdx.y * kT;
     const int li = lane & 31, kh = (lane >> 5) * 8;
+    const int64_t kb = (int64_t)blockIdx.z * KS;
+    const int64_t ke = (kb + KS < K) ? kb + KS : K;
+    C += (int64_t)blockIdx.z * slice;
     f32x16 acc;
 #pragma unroll
     for (int t = 0; t < 16; ++t) acc[t] = 0.f;
-    for (int64_t k0 = 0; k0 < K; k0 += kKs) {
+    for (int64_t k0 = kb; k0 < ke; k0 += kKs) {
         bf16x8 a, b;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
             const int64_t r = r0 + li, c = c0 + li, k = k0 + kh + j;
             float av = 0.f, bv = 0.f;
-            if (r < M && k < K) av = TA ? A[k * lda + r] : A[r * lda + k];
-            if (c < N && k < K) bv = TB ? B[c * ldb + k] : B[k * ldb + c];
+            if (r < M && k < ke) av = TA ? A[k * lda + r] : A[r * lda + k];
+            if (c < N && k < ke) bv = TB ? B[c * ldb + k] : B[k * ldb + c];
             a[j] = to_bf16(av);
             b[j] = to_bf16(bv);
         }
@@ -97,6 +106,17 @@ __global__ __launch_bounds__(64) void gemm_bf16_kernel(const float* __restrict__
     }
 }
 
+// C[r*ldc + c] = sum over the S slices of part (row-major M x N each), in
+// slice order.
+__global__ void splitk_sum_kernel(const float* __restrict__ part, int S, int64_t M, int64_t N, float* __restrict__ C,
+                                  int64_t ldc) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= M * N) return;
+    float s = 0.f;
+    for (int z = 0; z < S; ++z) s += part[(int64_t)z * M * N + t];
+    C[(t / N) * ldc + t % N] = s;
+}
+
 // Z = Y + bias (+ row-broadcast), H = relu(Z); Y may alias Z.
 __global__ void bias_relu_kernel(const float* Y, const float* bias, float* Z, float* H, int64_t N, int w) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -106,15 +126,28 @@ __global__ void bias_relu_kernel(const float* Y, const float* bias, float* Z, fl
     H[t] = z > 0.f ? z : 0.f;
 }
 
-// Column means of H over rows [0, ns) and [ns, ns + nt): cs, ct (w threads).
-__global__ void pool_kernel(const float* H, int64_t ns, int64_t nt, int w, float* cs, float* ct) {
+// Column sums of rows [row0, row1) of D (row-major, width w) in two fixed-
+// order stages: block b of colsum_part_kernel sums kSumRows rows per column
+// into part[b*w + c]; colsum_finish_kernel sums the parts in block order and
+// divides by div.
+constexpr int kSumRows = 64;
+__global__ void colsum_part_kernel(const float* __restrict__ D, int64_t row0, int64_t row1, int w,
+                                   float* __restrict__ part) {
+    const int c = threadIdx.x;
+    if (c >= w) return;
+    const int64_t r0 = row0 + (int64_t)blockIdx.x * kSumRows;
+    const int64_t r1 = (r0 + kSumRows < row1) ? r0 + kSumRows : row1;
+    float s = 0.f;
+    for (int64_t i = r0; i < r1; ++i) s += D[i * w + c];
+    part[(int64_t)blockIdx.x * w + c] = s;
+}
+__global__ void colsum_finish_kernel(const float* __restrict__ part, int nparts, int w, float div,
+                                     float* __restrict__ out) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= w) return;
-    float s = 0.f, t = 0.f;
-    for (int64_t i = 0; i < ns; ++i) s += H[i * w + c];
-    for (int64_t i = ns; i < ns + nt; ++i) t += H[i * w + c];
-    cs[c] = ns > 0 ? s / (float)ns : 0.f;
-    ct[c] = nt > 0 ? t / (float)nt : 0.f;
+    float s = 0.f;
+    for (int b = 0; b < nparts; ++b) s += part[(int64_t)b * w + c];
+    out[c] = s / div;
 }
 
 // g = cs W2b + ct W2c + b2 (w threads).
@@ -147,15 +180,6 @@ __global__ void head_loss_kernel(const float* Y, int64_t ldy, const float* b4, c
 __global__ void relu_back_kernel(float* dH, const float* Z, int64_t total) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < total && !(Z[t] > 0.f)) dH[t] = 0.f;
-}
-
-// Column sums of D (N x w) into out[w] (w threads; fixed row order).
-__global__ void colsum_kernel(const float* D, int64_t N, int w, float* out) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= w) return;
-    float s = 0.f;
-    for (int64_t i = 0; i < N; ++i) s += D[i * w + c];
-    out[c] = s;
 }
 
 // Context backward: with dg = colsum(dZ2): dW2b = cs^T dg, dW2c = ct^T dg,
@@ -289,6 +313,8 @@ struct pdot_mlp {
     float *W1p = nullptr, *dW1p = nullptr, *w4p = nullptr;  // padded W1 (16x128), its gradient, w4 as 128 x 32
     float *cs = nullptr, *ct = nullptr, *gctx = nullptr, *dcs = nullptr, *dct = nullptr, *dg = nullptr;
     float *zh = nullptr, *logw = nullptr;  // weight head pre-activations and output
+    float *sumpart = nullptr;              // column-sum parts (cap / kSumRows + 2 blocks of kW)
+    float *kpart = nullptr;                // split-K partial products (kMaxSlices x kW x kW)
     std::vector<float> hbuf;
     std::string err;
 };
@@ -311,8 +337,13 @@ int malloc_f(pdot_mlp* m, float** p, int64_t n) {
     return PDOT_OK;
 }
 
+constexpr int kMaxSlices = 64;      // split-K slices (partials of at most kW x kW)
+constexpr int64_t kSliceK = 256;    // k per slice, at least
+
 int ensure_cap(pdot_mlp* m, int64_t N) {
     if (N <= m->cap) return PDOT_OK;
+    if (m->sumpart) (void)hipFree(m->sumpart);
+    m->sumpart = nullptr;
     float** bufs[] = {&m->X0, &m->Z1, &m->H1, &m->Z2, &m->H2, &m->Z3, &m->H3, &m->D, &m->E};
     for (float** b : bufs)
         if (*b) (void)hipFree(*b);
@@ -330,6 +361,7 @@ int ensure_cap(pdot_mlp* m, int64_t N) {
     for (float** b : narrow)
         if ((rc = malloc_f(m, b, N))) return rc;
     if ((rc = malloc_f(m, &m->feat, N * kF))) return rc;
+    if ((rc = malloc_f(m, &m->sumpart, (N / kSumRows + 2) * kW))) return rc;
     m->cap = N;
     return PDOT_OK;
 }
@@ -337,8 +369,32 @@ int ensure_cap(pdot_mlp* m, int64_t N) {
 template <bool TA, bool TB>
 hipError_t gemm(pdot_mlp* m, const float* A, int64_t lda, const float* B, int64_t ldb, float* C, int64_t ldc,
                 int64_t M, int64_t N, int64_t K) {
-    dim3 grid((unsigned)((M + kT - 1) / kT), (unsigned)((N + kT - 1) / kT));
-    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), grid, dim3(64), 0, m->stream, A, lda, B, ldb, C, ldc, M, N, K);
+    const unsigned gx = (unsigned)((M + kT - 1) / kT), gy = (unsigned)((N + kT - 1) / kT);
+    // split K when the output has few tiles and K is long
+    int64_t S = 1;
+    if ((int64_t)gx * gy < 64 && M <= kW && N <= kW && K >= 2 * kSliceK)
+        S = std::min<int64_t>(kMaxSlices, (K + kSliceK - 1) / kSliceK);
+    if (S == 1) {
+        hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), dim3(gx, gy, 1), dim3(64), 0, m->stream, A, lda, B, ldb, C, ldc,
+                           M, N, K, K, (int64_t)0);
+        return hipGetLastError();
+    }
+    const int64_t KS = ((K + S - 1) / S + kKs - 1) / kKs * kKs;  // a multiple of the k-step
+    S = (K + KS - 1) / KS;
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, TB>), dim3(gx, gy, (unsigned)S), dim3(64), 0, m->stream, A, lda, B, ldb,
+                       m->kpart, N, M, N, K, KS, M * N);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(splitk_sum_kernel, dim3(blocks_for(M * N)), dim3(256), 0, m->stream, m->kpart, (int)S, M, N, C,
+                       ldc);
+    return hipGetLastError();
+}
+
+// Column sums (divided by div) of rows [row0, row1) of D (width w) into out.
+hipError_t colsum_rows(pdot_mlp* m, const float* D, int64_t row0, int64_t row1, int w, float div, float* out) {
+    const int64_t nb = std::max<int64_t>(1, (row1 - row0 + kSumRows - 1) / kSumRows);
+    hipLaunchKernelGGL(colsum_part_kernel, dim3((unsigned)nb), dim3(kW), 0, m->stream, D, row0, row1, w, m->sumpart);
+    hipLaunchKernelGGL(colsum_finish_kernel, dim3(1), dim3(kW), 0, m->stream, m->sumpart, (int)nb, w, div, out);
     return hipGetLastError();
 }
 
@@ -353,7 +409,8 @@ int forward_trunk(pdot_mlp* m, int64_t ns, int64_t nt) {
     MHIPCHK(m, (gemm<false, false>(m, m->X0, kFP, m->W1p, kW, m->Z1, kW, N, kW, kFP)));
     hipLaunchKernelGGL(bias_relu_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->Z1, P + m->ob1, m->Z1,
                        m->H1, N, kW);
-    hipLaunchKernelGGL(pool_kernel, dim3(1), dim3(kW), 0, m->stream, m->H1, ns, nt, kW, m->cs, m->ct);
+    MHIPCHK(m, colsum_rows(m, m->H1, 0, ns, kW, ns > 0 ? (float)ns : 1.f, m->cs));
+    MHIPCHK(m, colsum_rows(m, m->H1, ns, ns + nt, kW, nt > 0 ? (float)nt : 1.f, m->ct));
     MHIPCHK(m, hipGetLastError());
     return PDOT_OK;
 }
@@ -400,17 +457,17 @@ int backward(pdot_mlp* m, int64_t ns, int64_t nt) {
     MHIPCHK(m, hipMemsetAsync(G + m->oWha, 0, sizeof(float) * (m->np - m->oWha), m->stream));
     // head: dW4 = H3^T dout (128 x 1), db4 = sum dout, dH3 = dout w4^T
     MHIPCHK(m, (gemm<true, false>(m, m->H3, kW, m->dout, 1, G + m->ow4, 1, kW, 1, N)));
-    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(1), 0, m->stream, m->dout, N, 1, G + m->ob4);
+    MHIPCHK(m, colsum_rows(m, m->dout, 0, N, 1, 1.f, G + m->ob4));
     MHIPCHK(m, (gemm<false, true>(m, m->dout, 1, P + m->ow4, 1, m->D, kW, N, kW, 1)));
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->D, m->Z3, N * kW);
     // layer 3: dW3 = H2^T dZ3, db3 = colsum dZ3, dH2 = dZ3 W3^T
     MHIPCHK(m, (gemm<true, false>(m, m->H2, kW, m->D, kW, G + m->oW3, kW, kW, kW, N)));
-    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, G + m->ob3);
+    MHIPCHK(m, colsum_rows(m, m->D, 0, N, kW, 1.f, G + m->ob3));
     MHIPCHK(m, (gemm<false, true>(m, m->D, kW, P + m->oW3, kW, m->E, kW, N, kW, kW)));
     hipLaunchKernelGGL(relu_back_kernel, dim3(blocks_for(N * kW)), dim3(256), 0, m->stream, m->E, m->Z2, N * kW);
     // layer 2: dW2a = H1^T dZ2, dg = colsum dZ2 (= db2), context weights, dH1 = dZ2 W2a^T + pooling
     MHIPCHK(m, (gemm<true, false>(m, m->H1, kW, m->E, kW, G + m->oW2a, kW, kW, kW, N)));
-    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->E, N, kW, G + m->ob2);
+    MHIPCHK(m, colsum_rows(m, m->E, 0, N, kW, 1.f, G + m->ob2));
     hipLaunchKernelGGL(context_back_kernel, dim3(1), dim3(kW), 0, m->stream, m->cs, m->ct, P + m->oW2b, P + m->oW2c,
                        G + m->ob2, G + m->oW2b, G + m->oW2c, m->dcs, m->dct, kW);
     MHIPCHK(m, (gemm<false, true>(m, m->E, kW, P + m->oW2a, kW, m->D, kW, N, kW, kW)));
@@ -420,7 +477,7 @@ int backward(pdot_mlp* m, int64_t ns, int64_t nt) {
     // layer 1: dW1 = X0^T dZ1 (16 x 128, rows 0..3 kept), db1 = colsum dZ1
     MHIPCHK(m, (gemm<true, false>(m, m->X0, kFP, m->D, kW, m->dW1p, kW, kFP, kW, N)));
     MHIPCHK(m, hipMemcpyAsync(G + m->oW1, m->dW1p, sizeof(float) * kF * kW, hipMemcpyDeviceToDevice, m->stream));
-    hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, G + m->ob1);
+    MHIPCHK(m, colsum_rows(m, m->D, 0, N, kW, 1.f, G + m->ob1));
     MHIPCHK(m, hipGetLastError());
     return PDOT_OK;
 }
@@ -442,7 +499,7 @@ int backward_weight(pdot_mlp* m, int64_t ns, int64_t nt, float target, bool trai
         MHIPCHK(m, (gemm<true, false>(m, m->X0, kFP, m->D, kW, m->dW1p, kW, kFP, kW, N)));
         MHIPCHK(m, hipMemcpyAsync(m->g + m->oW1, m->dW1p, sizeof(float) * kF * kW, hipMemcpyDeviceToDevice,
                                   m->stream));
-        hipLaunchKernelGGL(colsum_kernel, dim3(1), dim3(kW), 0, m->stream, m->D, N, kW, m->g + m->ob1);
+        MHIPCHK(m, colsum_rows(m, m->D, 0, N, kW, 1.f, m->g + m->ob1));
     }
     MHIPCHK(m, hipGetLastError());
     return PDOT_OK;
@@ -494,6 +551,7 @@ int pdot_mlp_create(pdot_mlp** out, int32_t device) {
     if ((rc = malloc_f(m, &m->W1p, kFP * kW))) return rc;
     if ((rc = malloc_f(m, &m->dW1p, kFP * kW))) return rc;
     if ((rc = malloc_f(m, &m->w4p, kW * kT))) return rc;
+    if ((rc = malloc_f(m, &m->kpart, (int64_t)kMaxSlices * kW * kW))) return rc;
     float** sm[] = {&m->cs, &m->ct, &m->gctx, &m->dcs, &m->dct, &m->dg, &m->zh, &m->logw};
     for (float** q : sm)
         if ((rc = malloc_f(m, q, kW))) return rc;
@@ -506,7 +564,7 @@ void pdot_mlp_destroy(pdot_mlp* m) {
     if (m->stream) (void)hipStreamSynchronize(m->stream);
     float* all[] = {m->p, m->g, m->m, m->v, m->X0, m->Z1, m->H1, m->Z2, m->H2, m->Z3, m->H3, m->Y4, m->out, m->tgt,
                     m->dout, m->sq, m->D, m->E, m->feat, m->W1p, m->dW1p, m->w4p, m->cs, m->ct, m->gctx, m->dcs,
-                    m->dct, m->dg, m->zh, m->logw};
+                    m->dct, m->dg, m->zh, m->logw, m->sumpart, m->kpart};
     for (float* q : all)
         if (q) (void)hipFree(q);
     if (m->stream) (void)hipStreamDestroy(m->stream);

@@ -19,7 +19,9 @@ def net():
 
 @pytest.mark.parametrize("M_,N_,K_,ta,tb", [(32, 32, 16, 0, 0), (37, 45, 21, 0, 0), (64, 128, 128, 0, 0),
                                             (128, 128, 300, 1, 0), (300, 128, 128, 0, 1), (5, 3, 7, 1, 1),
-                                            (16, 128, 1000, 1, 0), (1000, 1, 128, 0, 0)])
+                                            (16, 128, 1000, 1, 0), (1000, 1, 128, 0, 0),
+                                            # split over K (few output tiles, long K), as the weight gradients
+                                            (128, 128, 8192, 1, 0), (64, 32, 5000, 0, 0), (128, 1, 8191, 1, 0)])
 def test_mfma_product_matches_emulation(net, M_, N_, K_, ta, tb):
     rng = np.random.default_rng(M_ + N_ + K_)
     A = rng.standard_normal((K_, M_) if ta else (M_, K_)).astype(np.float32)
