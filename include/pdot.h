@@ -226,6 +226,14 @@ int pdot_use_graphs(pdot_ctx* ctx, int32_t enable);
  * counterpart in the reference (a data-movement choice of this library). */
 int pdot_set_defer(pdot_ctx* ctx, int32_t enable);
 
+/* Passes that leave the average alone (every second ordinary pass with
+ * deferred averaging) run a kernel compiled for that kind, without the
+ * average's loads and registers, where the form has one (point clouds with
+ * FP32 storage); enable = 0 runs the kernel that decides at run time for
+ * every pass, for comparison.  Results are identical either way.  No
+ * counterpart in the reference (a kernel-selection choice of this library). */
+int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable);
+
 /* Launch geometry of the streaming step kernel: rows per block, column
  * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the
  * default choice (used by the launch-shape tuner); 0 keeps the default.

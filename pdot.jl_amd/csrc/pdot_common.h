@@ -66,6 +66,23 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 #ifndef PDOT_F32P_PIPE
 #define PDOT_F32P_PIPE 0  // the cost's registers leave no room for a prefetched batch
 #endif
+// Columns per batch and batches prefetched in the kernel of the passes that
+// leave the average alone (kKindLeave below), for the forms that have one;
+// the waves per block and the launch geometry are the generic kernel's, so
+// both kinds share one layout of the partial sums.
+#ifndef PDOT_F32P_LEAVE_UNROLL
+#define PDOT_F32P_LEAVE_UNROLL 4  // four columns per batch once the average's registers are free
+#endif
+#ifndef PDOT_F32P_LEAVE_PIPE
+#define PDOT_F32P_LEAVE_PIPE 0
+#endif
+
+// Kinds of an ordinary pass: kKindAny decides at run time whether it reads and
+// writes the running average; kKindLeave is a pass known to leave the average
+// alone (its update deferred to the next pass), compiled without the
+// average's loads, registers and stores.
+enum { kKindAny = 0, kKindLeave = 1 };
+
 // POINTS: the cost is formed from point clouds (more registers per lane, no
 // C stream), which has its own best shape.
 template <typename T, bool POINTS = false>
@@ -85,6 +102,22 @@ struct StepShape<double, true> {
 template <>
 struct StepShape<float, true> {
     static constexpr int kWaves = PDOT_F32P_WAVES, kUnroll = PDOT_F32P_UNROLL, kPipe = PDOT_F32P_PIPE;
+};
+// The leave-alone kind's shape, and whether a form has a kernel of its own for
+// it.  Measured at 10k x 10k per iteration (both kinds on the generic
+// kernel's geometry): FP32 point clouds 4% faster with it (142 -> 126
+// registers with four columns per batch); FP32 dense and FP64 point clouds
+// within 0.5%, FP64 dense no faster (418 us against 420 us for that kind of
+// pass), so those keep the one kernel.
+template <typename T, bool POINTS = false>
+struct LeaveShape {
+    static constexpr bool kOwnKernel = false;
+    static constexpr int kUnroll = StepShape<T, POINTS>::kUnroll, kPipe = StepShape<T, POINTS>::kPipe;
+};
+template <>
+struct LeaveShape<float, true> {
+    static constexpr bool kOwnKernel = true;
+    static constexpr int kUnroll = PDOT_F32P_LEAVE_UNROLL, kPipe = PDOT_F32P_LEAVE_PIPE;
 };
 
 // ------------------------------------------------------------------ helpers
@@ -497,9 +530,10 @@ struct TileArgs {
 // evaluation reductions of the input point and of its average.  Called by
 // all threads of the block; thread 0 receives ||Delta||^2 of the tile in
 // *dd_out and the kMatStats evaluation reductions in ev_out.
-template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE, int COST = kCostDense>
+template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE, int COST = kCostDense, int KIND = kKindAny>
 __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)[kChunk], double* red, double* dd_out,
                                           double* ev_out) {
+    static_assert(!(EVAL && KIND == kKindLeave), "an evaluation's pass always updates the average");
     constexpr int VEC = VecT<T>::N;
     using V = typename VecT<T>::type;
     constexpr int WROWS = kWave * VEC;
@@ -512,9 +546,10 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const bool lane_active = i0 < row_end;
     const bool avg_empty = a.avg_empty;
     const bool do_step = a.do_step;
-    // a deferring pass leaves the average alone (never an evaluation)
-    const bool skip_avg = avg_empty || (!EVAL && a.defer);
-    const bool pend = a.pend && !avg_empty;
+    // a deferring pass leaves the average alone (never an evaluation); so does
+    // every pass of the leave-alone kind, which has no lagging contribution
+    const bool skip_avg = (KIND == kKindLeave) || avg_empty || (!EVAL && a.defer);
+    const bool pend = (KIND != kKindLeave) && a.pend && !avg_empty;
     const double pnw = a.pnw, paw = a.paw;
     const T* __restrict__ C = a.C;
     const T* __restrict__ Xc = a.Xc;

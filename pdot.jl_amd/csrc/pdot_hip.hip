@@ -74,6 +74,7 @@ struct StepParams {
     int32_t TC, Cb, R;
     int32_t TRB;         // rows per row block: n split evenly over the R blocks (<= the block's rows)
     int32_t interleave;  // 1: block cb takes columns cb, cb+Cb, ...; 0: a contiguous range
+    int32_t* kind_err;   // set by a leave-alone pass whose state says it must update the average
 };
 
 struct ReduceParams {
@@ -95,10 +96,17 @@ struct ReduceParams {
 // One PDHG pass over the whole problem: block (rb, cb) owns rows
 // [rb*TRB, min(n, (rb+1)*TRB)) of the columns cb, cb+Cb, ... (interleaved) or of the
 // contiguous range [cb*TC, (cb+1)*TC); step_tile does the work.
-template <typename T, bool EVAL, int COST>
+// KIND = kKindLeave: a pass the host knows to leave the average alone (the
+// passes of a launch sequence alternate deterministically); should the state
+// say otherwise, the pass does nothing and flags kind_err, and the controller
+// then stops the solve instead of committing.
+template <typename T, bool EVAL, int COST, int KIND = kKindAny>
 __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave)) void step_kernel(StepParams p) {
     using Shape = StepShape<T, COST != kCostDense>;
+    using LShape = LeaveShape<T, COST != kCostDense>;
     constexpr int WAVES = Shape::kWaves;
+    constexpr int UNROLL = KIND == kKindLeave ? LShape::kUnroll : Shape::kUnroll;
+    constexpr int PIPE = KIND == kKindLeave ? LShape::kPipe : Shape::kPipe;
     constexpr int TR = WAVES * kWave * VecT<T>::N;
     __shared__ pdot_solver_state sst;
     __shared__ double red[WAVES * kWave];
@@ -108,6 +116,10 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     __syncthreads();
     const bool do_step = !sst.numerical_error;
     if (!do_step && !EVAL) return;
+    if (KIND == kKindLeave && !pass_defers(sst, false)) {
+        if (tid == 0 && blockIdx.x == 0) *p.kind_err = 1;
+        return;
+    }
 
     const int rb = blockIdx.x % p.R;
     const int cb = blockIdx.x / p.R;
@@ -155,7 +167,7 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     a.pnw = sst.pend_nw;
     a.paw = sst.pend_aw;
     double dd = 0.0, ev[kMatStats];
-    step_tile<T, EVAL, Shape::kUnroll, WAVES, Shape::kPipe, COST>(a, colbuf, red, &dd, ev);
+    step_tile<T, EVAL, UNROLL, WAVES, PIPE, COST, KIND>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
         if (do_step) p.sqpart[blockIdx.x] = dd;
         if (EVAL) {
@@ -263,13 +275,21 @@ __global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart,
 // left pending for the host's decision instead (one wave).  A halted solve
 // (numerical error) commits nothing.
 template <bool EVAL>
-__global__ __launch_bounds__(kWave) void controller_kernel(pdot_solver_state* st, const double* redpart, int32_t nred) {
+__global__ __launch_bounds__(kWave) void controller_kernel(pdot_solver_state* st, const double* redpart, int32_t nred,
+                                                           const int32_t* kind_err) {
     __shared__ pdot_solver_state sst;
     if (threadIdx.x == 0) sst = *st;
     __syncthreads();
     if (sst.numerical_error) return;
     if (EVAL) {
         if (threadIdx.x == 0) st->pending = 1;
+        return;
+    }
+    if (*kind_err) {  // the step kernel did not perform the pass (see step_kernel)
+        if (threadIdx.x == 0) {
+            st->numerical_error = 1;
+            st->error_call = sst.take_step_calls + 1;
+        }
         return;
     }
     reduce_and_commit(&sst, redpart, nred);
@@ -413,6 +433,8 @@ struct pdot_ctx {
     bool uploaded = false, started = false;
     bool use_graph = false;
     bool defer = true;  // ordinary passes alternate deferring / applying the average's update
+    bool pass_kernels = true;     // the passes that leave the average alone use their own kernel
+    int32_t* kind_err = nullptr;  // device flag, see step_kernel
     std::map<int64_t, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> events;
     int64_t bytes = 0;
@@ -477,12 +499,17 @@ int setup_geometry(pdot_ctx* ctx) {
     free_partials(ctx);  // evalvec, allocated last, stays null if an allocation below fails
     const int TR = rows_per_block(ctx->storage, ctx->cost_kind);
     const int64_t R = (ctx->n + TR - 1) / TR;
+    // One full wave of resident blocks: every block streams the same number
+    // of columns and no straggler round follows.  Both kinds of pass use this
+    // geometry (the generic kernel's occupancy): the leave-alone kernel's
+    // lower register count would fit more column blocks (2% faster for FP32
+    // storage), but the partial row sums would then be grouped differently
+    // in every second pass and the results would no longer equal those of
+    // the eager average bit for bit.
     int64_t Cb;
     if (ctx->cb_hint > 0) {
         Cb = ctx->cb_hint;
     } else {
-        // one full wave of resident blocks: every block streams the same
-        // number of columns and no straggler round follows
         Cb = ctx->resident_blocks / R;
     }
     if (Cb < 1) Cb = 1;
@@ -540,6 +567,7 @@ StepParams step_params(pdot_ctx* ctx) {
     p.Cb = ctx->Cb;
     p.R = ctx->R;
     p.interleave = ctx->interleave;
+    p.kind_err = ctx->kind_err;
     return p;
 }
 
@@ -567,25 +595,39 @@ ReduceParams reduce_params(pdot_ctx* ctx) {
     return p;
 }
 
-template <typename T>
-void launch_step_t(int cost, bool eval, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
-    if (cost == kCostSqEuclid) {
-        if (eval) hipLaunchKernelGGL((step_kernel<T, true, kCostSqEuclid>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((step_kernel<T, false, kCostSqEuclid>), grid, block, 0, stream, p);
-    } else if (cost == kCostL1) {
-        if (eval) hipLaunchKernelGGL((step_kernel<T, true, kCostL1>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((step_kernel<T, false, kCostL1>), grid, block, 0, stream, p);
+template <typename T, int COST>
+void launch_step_c(bool eval, bool leave, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
+    if (eval) hipLaunchKernelGGL((step_kernel<T, true, COST>), grid, block, 0, stream, p);
+    else if constexpr (LeaveShape<T, COST != kCostDense>::kOwnKernel) {
+        if (leave) hipLaunchKernelGGL((step_kernel<T, false, COST, kKindLeave>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((step_kernel<T, false, COST>), grid, block, 0, stream, p);
     } else {
-        if (eval) hipLaunchKernelGGL((step_kernel<T, true, kCostDense>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((step_kernel<T, false, kCostDense>), grid, block, 0, stream, p);
+        hipLaunchKernelGGL((step_kernel<T, false, COST>), grid, block, 0, stream, p);
     }
 }
 
-hipError_t launch_step(pdot_ctx* ctx, bool eval) {
+template <typename T>
+void launch_step_t(int cost, bool eval, bool leave, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
+    if (cost == kCostSqEuclid) launch_step_c<T, kCostSqEuclid>(eval, leave, grid, block, stream, p);
+    else if (cost == kCostL1) launch_step_c<T, kCostL1>(eval, leave, grid, block, stream, p);
+    else launch_step_c<T, kCostDense>(eval, leave, grid, block, stream, p);
+}
+
+// leave: the pass is known to leave the average alone (kKindLeave kernel
+// where the form has one and the per-kind kernels are enabled).
+bool uses_leave_kernel(pdot_ctx* ctx, bool eval, bool leave) {
+    if (!leave || eval || !ctx->pass_kernels) return false;
+    const bool points = ctx->cost_kind != kCostDense;
+    if (ctx->storage == PDOT_STORAGE_F64) return points ? LeaveShape<double, true>::kOwnKernel : LeaveShape<double, false>::kOwnKernel;
+    return points ? LeaveShape<float, true>::kOwnKernel : LeaveShape<float, false>::kOwnKernel;
+}
+
+hipError_t launch_step(pdot_ctx* ctx, bool eval, bool leave = false) {
+    leave = uses_leave_kernel(ctx, eval, leave);
     StepParams p = step_params(ctx);
     dim3 grid(ctx->nstep), block(step_waves(ctx->storage, ctx->cost_kind) * kWave);
-    if (ctx->storage == PDOT_STORAGE_F64) launch_step_t<double>(ctx->cost_kind, eval, grid, block, ctx->stream, p);
-    else launch_step_t<float>(ctx->cost_kind, eval, grid, block, ctx->stream, p);
+    if (ctx->storage == PDOT_STORAGE_F64) launch_step_t<double>(ctx->cost_kind, eval, leave, grid, block, ctx->stream, p);
+    else launch_step_t<float>(ctx->cost_kind, eval, leave, grid, block, ctx->stream, p);
     return hipGetLastError();
 }
 
@@ -596,8 +638,10 @@ hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
     else hipLaunchKernelGGL((reduce_kernel<false>), grid, block, 0, ctx->stream, p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (eval) hipLaunchKernelGGL((controller_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
-    else hipLaunchKernelGGL((controller_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred);
+    if (eval) hipLaunchKernelGGL((controller_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred,
+                                 ctx->kind_err);
+    else hipLaunchKernelGGL((controller_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred,
+                            ctx->kind_err);
     return hipGetLastError();
 }
 
@@ -606,13 +650,44 @@ hipError_t launch_commit(pdot_ctx* ctx) {
     return hipGetLastError();
 }
 
+// Whether pass k (from 0) of a launch sequence leaves the average alone.
+// With deferral, a committed pass that defers leaves the update pending and
+// the next applies it, so the kinds alternate from the first: the first
+// defers if the average is non-empty and holds no lagging contribution once a
+// pending evaluation's pass (which always updates the average) is committed.
+// A numerical error halts the remaining passes, whatever their kind.
+bool first_pass_leaves(const pdot_solver_state& st) {
+    if (!st.defer_ok) return false;
+    if (st.pending) return true;
+    return st.avg_pending == 0 && st.avg_count > 0;
+}
+bool pass_leaves(bool first_leaves, bool defer_ok, int64_t k) {
+    return defer_ok && ((k % 2 == 0) == first_leaves);
+}
+
 // A pass left pending by an evaluation is committed first; each later
 // pass commits itself in its reduce kernel.
-int enqueue_steps(pdot_ctx* ctx, int64_t count) {
+int enqueue_steps(pdot_ctx* ctx, int64_t count, bool first_leaves, bool defer_ok) {
     HIPCHK(ctx, launch_commit(ctx));
     for (int64_t k = 0; k < count; ++k) {
-        HIPCHK(ctx, launch_step(ctx, false));
+        const bool leave = pass_leaves(first_leaves, defer_ok, k);
+        HIPCHK(ctx, launch_step(ctx, false, leave));
         HIPCHK(ctx, launch_reduce(ctx, false));
+    }
+    return PDOT_OK;
+}
+
+// After a launch sequence: a leave-alone pass whose state said otherwise
+// (never expected) stopped the solve; report it.
+int check_pass_kinds(pdot_ctx* ctx) {
+    int32_t err = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&err, ctx->kind_err, sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    if (err) {
+        HIPCHK(ctx, hipMemsetAsync(ctx->kind_err, 0, sizeof(int32_t), ctx->stream));
+        HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+        return fail(ctx, PDOT_E_STATE, "internal: a pass launched as leaving the average alone had to update it; "
+                                       "the solve was stopped (numerical_error)");
     }
     return PDOT_OK;
 }
@@ -797,6 +872,8 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     if ((rc = dev_alloc(ctx, (void**)&ctx->b, sizeof(double) * (size_t)m))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalout, sizeof(double) * kEvalOut))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->stA, sizeof(pdot_solver_state)))) return rc;
+    if ((rc = dev_alloc(ctx, (void**)&ctx->kind_err, sizeof(int32_t)))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->kind_err, 0, sizeof(int32_t), ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_state, sizeof(pdot_solver_state), hipHostMallocDefault));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_eval, sizeof(double) * kEvalOut, hipHostMallocDefault));
     if ((rc = configure_kernel(ctx))) return rc;
@@ -810,7 +887,7 @@ void pdot_destroy(pdot_ctx* ctx) {
     free_graphs(ctx);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     free_partials(ctx);
-    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->stage};
+    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->stage};
     for (void* q : ps)
         if (q) hipFree(q);
     for (int s = 0; s < 4; ++s) {
@@ -1030,12 +1107,18 @@ int pdot_run_steps(pdot_ctx* ctx, int64_t count, pdot_solver_state* out) {
     if (!ctx->evalvec) return fail(ctx, PDOT_E_STATE, "pdot_run_steps: launch buffers missing after a failed allocation");
     if (count < 0) return fail(ctx, PDOT_E_ARG, "pdot_run_steps: negative count");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    pdot_solver_state st0;
+    int rc0 = read_state(ctx, &st0);
+    if (rc0) return rc0;
+    const bool first = first_pass_leaves(st0), dok = st0.defer_ok != 0;
     if (ctx->use_graph && count > 1) {
-        auto it = ctx->graphs.find(count);
+        // the kinds of the passes are part of the captured sequence
+        const int64_t key = count * 4 + (first ? 2 : 0) + (dok ? 1 : 0);
+        auto it = ctx->graphs.find(key);
         if (it == ctx->graphs.end()) {
             hipGraph_t g;
             HIPCHK(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            int rc = enqueue_steps(ctx, count);
+            int rc = enqueue_steps(ctx, count, first, dok);
             hipError_t e = hipStreamEndCapture(ctx->stream, &g);
             if (rc) return rc;
             if (e != hipSuccess) return fail(ctx, PDOT_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
@@ -1043,13 +1126,15 @@ int pdot_run_steps(pdot_ctx* ctx, int64_t count, pdot_solver_state* out) {
             e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
             if (e != hipSuccess) return fail(ctx, PDOT_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-            it = ctx->graphs.emplace(count, ge).first;
+            it = ctx->graphs.emplace(key, ge).first;
         }
         HIPCHK(ctx, hipGraphLaunch(it->second, ctx->stream));
     } else {
-        int rc = enqueue_steps(ctx, count);
+        int rc = enqueue_steps(ctx, count, first, dok);
         if (rc) return rc;
     }
+    int rc = check_pass_kinds(ctx);
+    if (rc) return rc;
     return read_state(ctx, out);
 }
 
@@ -1229,16 +1314,22 @@ int pdot_time_steps(pdot_ctx* ctx, int64_t count, double* step_kernel_ms, double
         ctx->events.push_back(e);
     }
     hipEvent_t* ev = ctx->events.data();
+    pdot_solver_state st0;
+    int rc0 = read_state(ctx, &st0);
+    if (rc0) return rc0;
+    const bool first = first_pass_leaves(st0), dok = st0.defer_ok != 0;
     HIPCHK(ctx, hipEventRecord(ev[0], ctx->stream));
     HIPCHK(ctx, launch_commit(ctx));
     for (int64_t k = 0; k < count; ++k) {
         HIPCHK(ctx, hipEventRecord(ev[2 + 2 * k], ctx->stream));
-        HIPCHK(ctx, launch_step(ctx, false));
+        const bool leave = pass_leaves(first, dok, k);
+        HIPCHK(ctx, launch_step(ctx, false, leave));
         HIPCHK(ctx, hipEventRecord(ev[3 + 2 * k], ctx->stream));
         HIPCHK(ctx, launch_reduce(ctx, false));
     }
     HIPCHK(ctx, hipEventRecord(ev[1], ctx->stream));
     HIPCHK(ctx, hipEventSynchronize(ev[1]));
+    if ((rc0 = check_pass_kinds(ctx))) return rc0;
     double ks = 0.0;
     for (int64_t k = 0; k < count; ++k) {
         float t = 0.f;
@@ -1270,6 +1361,16 @@ int pdot_set_defer(pdot_ctx* ctx, int32_t enable) {
     if (!enable && (rc = materialize_average(ctx, st))) return rc;
     st.defer_ok = enable;
     return write_state(ctx, &st);
+}
+
+int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable) {
+    if (!ctx) return PDOT_E_ARG;
+    if (enable != 0 && enable != 1) return fail(ctx, PDOT_E_ARG, "pdot_set_pass_kernels: enable must be 0 or 1");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pass_kernels = enable != 0;
+    free_graphs(ctx);  // captured sequences hold the kernels
+    return PDOT_OK;
 }
 
 int pdot_set_launch(pdot_ctx* ctx, int32_t col_blocks_hint, int32_t interleave) {

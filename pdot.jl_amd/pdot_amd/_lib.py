@@ -111,6 +111,7 @@ SIGNATURES = {
     "pdot_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
     "pdot_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_pass_kernels": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
@@ -312,6 +313,9 @@ class DeviceSolver:
     def set_defer(self, enable: bool) -> None:
         """Deferred averaging on (default) or off; results are identical either way."""
         self._check(self.lib.pdot_set_defer(self.h, int(bool(enable))), "pdot_set_defer")
+
+    def set_pass_kernels(self, enable: bool) -> None:
+        self._check(self.lib.pdot_set_pass_kernels(self.h, int(bool(enable))), "pdot_set_pass_kernels")
 
     def set_launch(self, col_blocks_hint: int = 0, interleave: bool = True) -> None:
         self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint), int(bool(interleave))), "pdot_set_launch")

@@ -277,3 +277,46 @@ def test_deferred_policy_variants_equal_eager_bitwise(kind):
             assert ca.primal_objective == cb.primal_objective and ca.l2_dual_residual == cb.l2_dual_residual
     _same_arrays((a.primal_solution, a.dual_source_solution, a.dual_target_solution),
                  (b.primal_solution, b.dual_source_solution, b.dual_target_solution))
+
+
+PASS_KERNEL_CASES = CASES + [("dense", _lib.STORAGE_F64, lambda: instances.l1_mixture_3d(210, 333, seed=15)),
+                             ("points", _lib.STORAGE_F32, lambda: instances.l1_mixture_3d_points(333, 210, seed=16))]
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+@pytest.mark.parametrize("form,storage,make", PASS_KERNEL_CASES)
+def test_pass_kernels_equal_run_time_kernel_bitwise(form, storage, make, graphs):
+    """The kernels compiled for the passes that leave the average alone
+    (chosen by the host from the alternation of the pass kinds, also inside
+    captured graphs) give exactly the results of the kernel that decides per
+    pass at run time: whole solves under restarts, a warm start and a
+    constant step size, and the final points."""
+    from dataclasses import replace
+    prob = make()
+    n = len(prob.source_distribution)
+    rng = np.random.default_rng(n)
+    warm = (0.01 * rng.standard_normal(n), 0.01 * rng.standard_normal(len(prob.target_distribution)))
+    base = P.default_parameters(1e-4, iteration_limit=400000)
+    runs = [("default", base, None), ("warm start", base, warm),
+            ("constant step", replace(base, step_size_policy_params=P.ConstantStepsizeParams()), None)]
+    for name, params, ws in runs:
+        outs = []
+        for pass_kernels in (True, False):
+            dev = _device(prob, storage, True, form == "points")
+            dev.set_pass_kernels(pass_kernels)
+            out = P.optimize(params, prob, storage=storage, solver=dev, warm_start=ws, use_graphs=graphs)
+            outs.append((out, dev.download(_lib.POINT_AVERAGE), dev.download(_lib.POINT_CURRENT), dev.get_state()))
+            dev.close()
+        (a, avg_a, cur_a, st_a), (b, avg_b, cur_b, st_b) = outs
+        assert a.termination_reason == b.termination_reason, name
+        assert a.iteration_count == b.iteration_count > 10, name
+        assert [s.restart_used for s in a.iteration_stats] == [s.restart_used for s in b.iteration_stats], name
+        for sa, sb in zip(a.iteration_stats, b.iteration_stats):
+            assert sa.step_size == sb.step_size and sa.primal_weight == sb.primal_weight, name
+            ca, cb = sa.convergence_information[0], sb.convergence_information[0]
+            assert ca.primal_objective == cb.primal_objective and ca.l2_dual_residual == cb.l2_dual_residual, name
+        _same_arrays((a.primal_solution, a.dual_source_solution, a.dual_target_solution),
+                     (b.primal_solution, b.dual_source_solution, b.dual_target_solution))
+        _same_arrays(avg_a + cur_a, avg_b + cur_b)
+        _same_state(st_a, st_b)
+        assert st_a.numerical_error == 0
