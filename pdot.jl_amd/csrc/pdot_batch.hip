@@ -56,12 +56,20 @@ constexpr int kBPipe = PDOT_BATCH_PIPE;  // column batches prefetched ahead in a
 #endif
 constexpr int kBUnroll = PDOT_BATCH_UNROLL;  // columns per load batch (1 slower, 4 no faster)
 constexpr int kBBlock = kWave * kBWaves;
-constexpr int kBRedWaves = 4;              // batched reduce kernel: waves per block (8 measured no faster)
+// batched reduce kernel: waves per block (a chunk with many partials is
+// split over them).  8 against 4: 16.3 against 18.1 us per pass with one
+// instance left, all-active passes unchanged, the 512-instance batch
+// 32.1 s against 33.2 s; 16 slower.
+constexpr int kBRedWaves = 8;
 constexpr int kBRedBlock = kWave * kBRedWaves;
 constexpr int kMinColTiles = 32;           // column tiles per instance (at least)
 constexpr int kHeavyParts = 32;            // chunks with more partials are reduced by a whole block
 
-constexpr int kMinTileCols = 4;            // columns per tile (at least; 8 measured slower)
+// columns per tile, at least.  8 makes tail passes about 0.5 us faster, but
+// the tail instances' partial sums group differently and the slowest
+// instance of the 512-instance batch then needs 824,448 iterations instead
+// of 651,904 (its restart decisions are that sensitive to rounding).
+constexpr int kMinTileCols = 4;
 
 struct BInst {
     int64_t n, m, ld;

@@ -70,16 +70,24 @@ def test_point_cloud_problem_validation():
 
 def test_calibrated_primal_weight_is_stored_with_the_network():
     """The committed weights carry the calibration of the initial primal
-    weight: the stored weight is the grid value with the fewest total
-    iterations over the calibration solves, and it beats the default 1.0."""
+    weight and the weight head's selection, consistent with the committed
+    training report: the constant is a grid value (not the reference's 1.0)
+    chosen over at least 20,000 grid solves, and the head is used only if it
+    needs at most SELECT_MARGIN times the constant's validation iterations."""
+    import json
+    from pdot_amd import train_warm_start as T
     meta = learned.load_meta()
+    report = json.load(open(os.path.join(os.path.dirname(learned.DEFAULT_WEIGHTS), "warm_start_report.json")))
     w = learned.load_initial_primal_weight()
-    grid = [float(g) for g in meta["calibration_grid"]]
-    totals = meta["calibration_iterations"].sum(axis=1)
-    assert w in grid
-    assert totals[grid.index(w)] == totals.min()
-    assert totals[grid.index(w)] < totals[grid.index(1.0)]
-    assert int(meta["calibration_seed0"]) == 200000
+    grid = [float(g) for g in meta["weight_grid"]]
+    assert w in grid and 1.0 not in grid
+    assert w == report["calibrated_constant"]
+    assert int(meta["weight_grid_solves"]) == report["grid_solves"] >= 20000
+    selected = bool(int(meta["weight_head_selected"]))
+    assert selected == report["head_selected"]
+    head, const = float(meta["weight_val_iterations_head"]), float(meta["weight_val_iterations_constant"])
+    assert (head, const) == (report["val_iterations_head"], report["val_iterations_constant"])
+    assert selected == (head <= T.SELECT_MARGIN * const)
 
 
 def test_point_third_evaluation_known_answer_matches_oracle():
