@@ -19,13 +19,18 @@ each with the roofline of its dominant kernel:
   c1_cpu_1000        1000 x 1000 uniform points, FP64, solved to 1e-4 on the
                      host cores by the literal restatement (and on the GPU);
   c2_f32_1e-6        the headline instance in FP32 storage solved to 1e-6;
+  c2_precision       FP32 storage against the FP64 oracle: the FP64 solve to
+                     1e-8 and the FP32-storage solve to 1.5e-8 on the same
+                     family at 1000 x 1000 (the 10k pair takes 15 minutes:
+                     --workload precision), and the bound that storing C in
+                     FP32 puts on the 10k optimum's change;
   c3_l1_mixture_30k  30000 x 30000 L1 mixture, dense FP32 C / FP64 duals, and
                      the FP64 form for the objective error, solved to 1e-4;
   c4_batched_512     512 instances of n, m in [64, 2048], FP32, to 1e-4, at
                      the reference's initial primal weight and the calibrated one;
   c5_warm_start_4096 held-out 4096 x 4096 Gaussian-mixture instances: zero
                      start against the learned starting states.
-Together about four minutes on one MI355X; --no-extra skips them.
+Together about four and a half minutes on one MI355X; --no-extra skips them.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-solve] [--no-cpu] [--no-extra]
 """
@@ -478,21 +483,19 @@ def bench_mlp(args) -> int:
     return 0
 
 
-def bench_precision(args) -> int:
-    """FP32 storage against the FP64 oracle at 10000 x 10000 (gaussian shift):
-    the FP64 solve tightened to 1e-8 and the FP32-storage solve to 1.5e-8,
-    each with the progress table printed every 50 evaluations.  About 15
-    minutes on one MI355X."""
+def tight_pair(n: int, verbosity: int = 0) -> dict:
+    """FP32 storage against the FP64 oracle on the n x n Gaussian-shift
+    instance: the FP64 solve tightened to 1e-8 and the FP32-storage solve to
+    1.5e-8; value = the relative difference of the two objectives <C, X>."""
     import numpy as np
     import pdot_amd as P
     from pdot_amd import _lib, instances
-    n = args.n
     prob = instances.gaussian_shift(n)
     out = {"metric": "objective of FP32 storage solved to 1.5e-8 vs the FP64 solve to 1e-8 (relative)",
            "unit": "relative", "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
            "config": {"workload": f"gaussian_shift {n}x{n}", "n": n, "m": n}}
     for name, storage, tol in (("f64_oracle", _lib.STORAGE_F64, 1e-8), ("f32", _lib.STORAGE_F32, 1.5e-8)):
-        params = P.default_parameters(tol, verbosity=3, record_iteration_stats=True)
+        params = P.default_parameters(tol, verbosity=verbosity, record_iteration_stats=verbosity > 0)
         t0 = time.time()
         sol = P.optimize(params, prob, storage=storage)
         wall = time.time() - t0
@@ -502,11 +505,31 @@ def bench_precision(args) -> int:
                      "objective": float(np.sum(prob.cost_matrix * sol.primal_solution))}
         sys.stdout.flush()
     out["value"] = abs(out["f32"]["objective"] - out["f64_oracle"]["objective"]) / abs(out["f64_oracle"]["objective"])
-    print(json.dumps(out))
+    return out
+
+
+def storage_bound(prob, objective: float) -> dict:
+    """How far storing C in FP32 can move the optimum: with masses summing to
+    1, every feasible plan has sum(X) = 1, so |OPT(fl32(C)) - OPT(C)| <=
+    max |C - fl32(C)|; relative to the FP64 objective given."""
+    import numpy as np
+    C = prob.cost_matrix
+    d = 0.0
+    for j0 in range(0, C.shape[1], 1000):
+        blk = C[:, j0:j0 + 1000]
+        d = max(d, float(np.abs(blk - blk.astype(np.float32).astype(np.float64)).max()))
+    return {"max_abs_c_minus_fl32_c": d, "objective": objective, "relative_bound": d / abs(objective)}
+
+
+def bench_precision(args) -> int:
+    """FP32 storage against the FP64 oracle at 10000 x 10000 (gaussian shift),
+    each solve printing its progress table every 50 evaluations.  About 15
+    minutes on one MI355X."""
+    print(json.dumps(tight_pair(args.n, verbosity=3)))
     return 0
 
 
-def extras(args, prob, f32_roofline) -> dict:
+def extras(args, prob, f32_roofline, f64_objective=None) -> dict:
     """The other workloads' entries of the default run, each printed to stderr
     as it finishes."""
     from pdot_amd import _lib
@@ -527,6 +550,17 @@ def extras(args, prob, f32_roofline) -> dict:
                   "tolerance": 1e-6, "value": r["seconds"], "roofline": f32_roofline})
         return r
     step("c2_f32_1e-6", c2)
+
+    def precision():
+        # the 10k pair takes about 15 minutes (--workload precision); here the same family at
+        # 1000 x 1000 (about 25 s), and the storage bound on the 10k instance itself
+        r = tight_pair(1000)
+        r["metric"] = ("FP32 storage vs the FP64 oracle: objective of the 1.5e-8 FP32-storage solve vs the "
+                       "1e-8 FP64 solve at 1000x1000 (relative), and the bound on the 10k instance")
+        r["storage_bound_10k"] = storage_bound(prob, f64_objective)
+        return r
+    if f64_objective is not None:
+        step("c2_precision", precision)
     step("c3_l1_mixture_30k", lambda: l1_mixture(30000, max(10, args.steps // 2), args.warmup, point_cloud=False))
     step("c4_batched_512", lambda: batched(args.steps))
     step("c5_warm_start_4096", lambda: warm_start_table(args.instances))
@@ -541,7 +575,7 @@ def main() -> int:
     ap.add_argument("--n", type=int, default=10000)
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves (and the extras)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
-    ap.add_argument("--no-extra", action="store_true", help="skip the c1, c2 1e-6, c3, c4 and c5 entries")
+    ap.add_argument("--no-extra", action="store_true", help="skip the c1, c2 1e-6, c2 precision, c3, c4 and c5 entries")
     ap.add_argument("--cpu-iters", type=int, default=80, help="iteration limit of the CPU baseline sample (about 15 s)")
     ap.add_argument("--instances", type=int, default=8, help="held-out instances of the warm-start workload")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -656,7 +690,7 @@ def main() -> int:
     if copy_probe:
         line["copy_probe"] = copy_probe
     if not (args.no_solve or args.no_extra) and n == 10000:
-        line["workloads"] = extras(args, prob, f32_roof)
+        line["workloads"] = extras(args, prob, f32_roof, (solves.get("f64") or {}).get("objective"))
     print(json.dumps(line))
     return 0
 
