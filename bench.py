@@ -334,6 +334,10 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
                           (when the weights file carries that head);
       network_duals       the network's predicted duals as the starting iterate
                           (forced, no acceptance test) with the calibrated weight;
+      exact_duals         the instance's own duals from the calibrated-weight
+                          solve as the starting iterate, with the same weight:
+                          what a perfect dual predictor would give (the time
+                          of the solve that found them is not counted);
       learned_start       learned.solve_warm: everything the learned state offers,
                           behind its KKT acceptance test and fallback.
     Prediction time is included in the learned rows.  A row's solve is capped
@@ -353,12 +357,12 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
         dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
         row = {"seed": 900200 + k}
 
-        def run(name, weight, duals=None, extra_sec=0.0, cap=None):
+        def run(name, weight, duals=None, extra_sec=0.0, cap=None, duals_out=None):
             from dataclasses import replace
             prm = replace(P.default_parameters(1e-4, record_iteration_stats=False), primal_importance=float(weight))
             if cap is not None:
                 prm.termination_criteria.iteration_limit = int(cap)
-            r = solve_to_tol_params(p, _lib.STORAGE_F32, prm, dev, warm_start=duals)
+            r = solve_to_tol_params(p, _lib.STORAGE_F32, prm, dev, warm_start=duals, duals_out=duals_out)
             r["seconds"] += extra_sec
             r["primal_importance"] = float(weight)
             row[name] = r
@@ -366,8 +370,9 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
 
         z = run("default_weight", 1.0)
         cap = limit_factor * max(z["iterations"], 128)
+        found = []
         if w_cal is not None:
-            run("calibrated_weight", w_cal, cap=cap)
+            run("calibrated_weight", w_cal, cap=cap, duals_out=found)
         t0 = time.time()
         u0, v0 = net.predict_duals(inst.x, inst.y, p.source_distribution, p.target_distribution)
         t_pred = time.time() - t0
@@ -377,6 +382,7 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
             run("network_weight", w_net, extra_sec=time.time() - t0, cap=cap)
         if w_cal is not None:
             run("network_duals", w_cal, duals=(u0, v0), extra_sec=t_pred, cap=cap)
+            run("exact_duals", w_cal, duals=found[0], cap=cap)
         dev.close()
         t0 = time.time()
         out, info = learned.solve_warm(P.default_parameters(1e-4, record_iteration_stats=False), p, inst.x, inst.y,
@@ -422,11 +428,14 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
             "rows": rows}
 
 
-def solve_to_tol_params(prob, storage, params, dev, warm_start=None) -> dict:
+def solve_to_tol_params(prob, storage, params, dev, warm_start=None, duals_out=None) -> dict:
+    """One solve; duals_out (a list), if given, receives the returned duals (u, v)."""
     import pdot_amd as P
     prof = P.SolveProfile()
     out = P.optimize(params, prob, storage=storage, solver=dev, profile=prof, warm_start=warm_start)
     ci = out.iteration_stats[-1].convergence_information[0]
+    if duals_out is not None:
+        duals_out.append((out.dual_source_solution.copy(), out.dual_target_solution.copy()))
     return {"seconds": prof.solve_sec, "iterations": out.iteration_count, "termination": out.termination_string,
             "objective": ci.primal_objective, "rel_kkt": P.relative_kkt_error(ci)}
 
