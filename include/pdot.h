@@ -112,8 +112,12 @@ int pdot_upload(pdot_ctx* ctx, const double* C, const double* a, const double* b
  * (PDOT_COST_SQEUCLID) or |t| (PDOT_COST_L1), summed from 0 in d order, and
  * s = max_ij of the raw sum if `normalize`, else 1 -- bit for bit the matrix a
  * dense caller builds that way, so results equal those of pdot_upload of
- * that matrix.  x: n x dim, y: m x dim (row-major), dim 1..3.  One n*m
- * stream per pass fewer than the dense form; no n*m allocation for C. */
+ * that matrix (FP64 storage).  With FP32 storage the division by s is a
+ * multiply by 1/s, within about one FP64 ulp of the quotient (the dense FP32
+ * form stores C rounded to FP32, 2^29 times coarser); a normalising maximum
+ * that is finite and outside [2^-1022, 2^1022] is then rejected (PDOT_E_ARG).
+ * x: n x dim, y: m x dim (row-major), dim 1..3.  One n*m stream per pass
+ * fewer than the dense form; no n*m allocation for C. */
 int pdot_upload_points(pdot_ctx* ctx, int32_t cost_kind, int32_t dim, const double* x, const double* y,
                        int32_t normalize, const double* a, const double* b);
 

@@ -465,8 +465,11 @@ constexpr int kMaxDim = 3;
 // residuals a - q*s could fall below the subnormal granularity 2^-1074 (the
 // relative bound alone underflows when s < 2^-122).
 __device__ __forceinline__ double exact_quotient(double a, double s, double y) {
-    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-960 && a >= 0x1p-900 * s && a <= 0x1p+900 * s))
-        return a / s;
+    // the bounds depend on s only (loop-invariant in the step kernels):
+    // a in [max(2^-960, 2^-900 s), 2^900 s], which a NaN fails
+    const bool s_ok = s >= 0x1p-1000 && s <= 0x1p+1000;
+    const double lo = fmax(0x1p-960, 0x1p-900 * s), hi = 0x1p+900 * s;
+    if (!(s_ok && a >= lo && a <= hi)) return a / s;
     const double q0 = a * y;
     const double r0 = fma(-q0, s, a);
     const double q1 = fma(r0, y, q0);
@@ -474,19 +477,34 @@ __device__ __forceinline__ double exact_quotient(double a, double s, double y) {
     return fma(r1, y, q1);
 }
 
-// crcp = 1.0 / cdiv (the IEEE division, once per thread)
-template <int COST>
+// crcp = 1.0 / cdiv (the IEEE division, once per thread).  EXACT: the
+// quotient is the correctly rounded one, so the costs equal the dense
+// generators' FP64 matrix bit for bit (FP64 storage).  Otherwise (FP32
+// storage, whose dense form stores C rounded to FP32) the quotient is the
+// product c * crcp, within about one FP64 ulp of c / cdiv, 2^29 times finer
+// than the FP32 rounding, with no branch per element.  It equals c / cdiv
+// for cdiv = 1, 0, +Inf and NaN; pdot_upload_points rejects the other scales
+// whose reciprocal is not a normal number (finite cdiv outside
+// [2^-1022, 2^1022]) for FP32 storage.
+template <int COST, bool EXACT>
 __device__ __forceinline__ double point_cost(const double* xi, const double* yj, int dim, double cdiv, double crcp) {
-    double c = 0.0;
+    // the sum starts from the first term: 0.0 + t*t and 0.0 + |t| equal the
+    // term itself in every case (neither is ever -0.0), so the dense
+    // generators' sum from 0.0 is reproduced without its first addition
+    double c = (COST == kCostSqEuclid) ? (yj[0] - xi[0]) * (yj[0] - xi[0]) : fabs(yj[0] - xi[0]);
 #pragma unroll
-    for (int d = 0; d < kMaxDim; ++d) {
+    for (int d = 1; d < kMaxDim; ++d) {
         if (d < dim) {
             const double t = yj[d] - xi[d];
             c += (COST == kCostSqEuclid) ? t * t : fabs(t);
         }
     }
-    // x / 1 == x: no division for unscaled costs
-    return cdiv == 1.0 ? c : exact_quotient(c, cdiv, crcp);
+    if constexpr (EXACT) {
+        // x / 1 == x: no division for unscaled costs
+        return cdiv == 1.0 ? c : exact_quotient(c, cdiv, crcp);
+    } else {
+        return c * crcp;
+    }
 }
 
 // Everything one block needs to advance one tile of one problem by a PDHG
@@ -635,7 +653,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                     unpack<V, T>(B.cv[q], c);
                 } else {
 #pragma unroll
-                    for (int e = 0; e < VEC; ++e) c[e] = point_cost<COST>(xi[e], B.yj[q], a.dim, a.cdiv, crcp);
+                    for (int e = 0; e < VEC; ++e) c[e] = point_cost<COST, sizeof(T) == 8>(xi[e], B.yj[q], a.dim, a.cdiv, crcp);
                 }
                 if (!skip_avg) {
                     unpack<V, T>(B.av[q], xa);

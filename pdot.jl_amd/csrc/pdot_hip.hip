@@ -351,7 +351,7 @@ __global__ __launch_bounds__(256) void point_stats_kernel(const double* __restri
         for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
             double xi[kMaxDim];
             for (int d = 0; d < kMaxDim; ++d) xi[d] = d < dim ? px[(int64_t)d * n + i] : 0.0;
-            const double x = point_cost<COST>(xi, yj, dim, cdiv, crcp);
+            const double x = point_cost<COST, true>(xi, yj, dim, cdiv, crcp);
             mx = nanmax(mx, fabs(x));
             ss += x * x;
         }
@@ -1039,6 +1039,10 @@ int pdot_upload_points(pdot_ctx* ctx, int32_t cost_kind, int32_t dim, const doub
         double mx = 0.0;
         for (size_t t = 0; t < parts.size(); t += 2) mx = host_nanmax(mx, parts[t]);
         if (!(mx > 0.0)) return fail(ctx, PDOT_E_ARG, "pdot_upload_points: cannot normalise a zero cost");
+        // FP32 storage scales the cost as raw * (1 / max), which needs a normal reciprocal
+        if (ctx->storage == PDOT_STORAGE_F32 && std::isfinite(mx) && (mx < 0x1p-1022 || mx > 0x1p+1022))
+            return fail(ctx, PDOT_E_ARG,
+                        "pdot_upload_points: FP32 storage needs the normalising maximum cost in [2^-1022, 2^1022]");
         cdiv = mx;
     }
     if ((rc = stats_pass(cdiv))) return rc;

@@ -8,8 +8,9 @@
 #include <stdint.h>
 
 static double quotient(double a, double s, double y) {
-    if (!(s >= 0x1p-1000 && s <= 0x1p+1000 && a >= 0x1p-960 && a >= 0x1p-900 * s && a <= 0x1p+900 * s))
-        return a / s;
+    const int s_ok = s >= 0x1p-1000 && s <= 0x1p+1000;
+    const double lo = fmax(0x1p-960, 0x1p-900 * s), hi = 0x1p+900 * s;
+    if (!(s_ok && a >= lo && a <= hi)) return a / s;
     const double q0 = a * y;
     const double r0 = fma(-q0, s, a);
     const double q1 = fma(r0, y, q0);

@@ -52,6 +52,31 @@ def test_argument_errors():
         _lib.DeviceSolver(0, 10)
 
 
+@pytest.mark.parametrize("scale", [1e308, 1e-309, 1.0])
+def test_fp32_point_costs_need_a_normal_reciprocal_of_the_scale(scale):
+    """FP32 storage forms the normalised point cost as raw * (1 / max): a
+    finite maximum outside [2^-1022, 2^1022] (here L1 costs near 1e308 and
+    near 1e-309, a subnormal) is rejected there; FP64 storage divides and
+    accepts it."""
+    g = np.random.default_rng(5)
+    x, y = g.random((40, 1)) * scale, g.random((30, 1)) * scale
+    prob = P.PointCloudProblem(x, y, np.ones(40) / 40, np.ones(30) / 30, "l1", normalize=True)
+    mx = np.abs(y[:, 0][None, :] - x[:, 0][:, None]).max()
+    assert np.isfinite(mx) and mx > 0
+    out_of_range = not (2.0 ** -1022 <= mx <= 2.0 ** 1022)
+    assert out_of_range == (scale != 1.0)
+    d32 = _lib.DeviceSolver(40, 30, storage=_lib.STORAGE_F32)
+    if out_of_range:
+        with pytest.raises(_lib.PdotError, match=r"FP32 storage needs the normalising maximum"):
+            d32.upload_points(prob)
+    else:
+        d32.upload_points(prob)
+    d32.close()
+    d64 = _lib.DeviceSolver(40, 30, storage=_lib.STORAGE_F64)
+    d64.upload_points(prob)
+    d64.close()
+
+
 def test_round_matrix_rejects_bad_shapes():
     prob = instances.uniform_sqeuclid(10, 8, seed=1)
     with pytest.raises(ValueError):

@@ -3,8 +3,9 @@ formed in registers each pass from the point clouds instead of being read
 from HBM.  The formula reproduces the dense generators' matrix bit for bit,
 so in FP64 a point-cloud solve must equal the dense solve of that matrix bit
 for bit (iterations, restarts, plan and duals); with FP32 storage (where the
-dense form stores C in FP32 and the point form keeps it exact) results are
-pinned against the HiGHS optima."""
+dense form stores C rounded to FP32 and the point form keeps it to about one
+FP64 ulp, multiplying by the reciprocal of the scale) results are pinned
+against the HiGHS optima."""
 import json
 import os
 
