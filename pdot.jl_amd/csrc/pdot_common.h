@@ -364,16 +364,73 @@ __device__ inline void controller_commit(pdot_solver_state& st, double dot, doub
     }
 }
 
+// Publishing a block's results to the last block of the same kernel without
+// a device-scope fence (on gfx950 an L2 write-back, and an invalidate on the
+// reading side): the results are written with agent-scope stores (written
+// through to the agent's coherence point), the writer waits until its
+// memory operations have completed, and only then counts itself done with an
+// agent-scope atomic; the block that counts last reads the results with
+// agent-scope loads.  This is the release sequence without the L2 write-back,
+// which only plain stores need.
+__device__ __forceinline__ void publish_f64(double* p, double v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double acquire_f64(const double* p) {
+    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// Waits until this wave's earlier memory operations have completed; the
+// signal fences keep the compiler from moving memory operations across it.
+__device__ __forceinline__ void wait_memory_ops() {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);  // vmcnt(0) expcnt(0) lgkmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+// Counts one publisher done (after wait_memory_ops); true for the last of
+// `total`, which also resets the counter for the next launch.
+__device__ __forceinline__ bool count_done(uint32_t* counter, uint32_t total) {
+    const uint32_t prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (prev != total - 1) return false;
+    __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
 // Reduce the per-block scalars of the reduce kernel (dot, ||dy||^2, ||dX||^2
 // chunk) in a fixed order and commit.  Called by the 64 lanes of one wave;
-// lane 0 updates *sst.
+// lane 0 updates *sst.  PUBLISHED: the scalars were written with
+// publish_f64 in the same kernel and are read with acquire_f64.
+template <bool PUBLISHED = false>
 __device__ inline void reduce_and_commit(pdot_solver_state* sst, const double* redpart, int nred) {
     const int lane = threadIdx.x & (kWave - 1);
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
-    for (int k = lane; k < nred; k += kWave) {
-        d0 += redpart[3 * k + 0];
-        d1 += redpart[3 * k + 1];
-        d2 += redpart[3 * k + 2];
+    if constexpr (PUBLISHED) {
+        // the loads of up to kU blocks per lane are issued before the first
+        // sum (one memory round trip for up to 64 kU blocks); the sums run
+        // in the same order as below
+        constexpr int kU = 8;
+        for (int k0 = lane; k0 < nred; k0 += kWave * kU) {
+            double t0[kU], t1[kU], t2[kU];
+#pragma unroll
+            for (int q = 0; q < kU; ++q) {
+                const int k = k0 + q * kWave;
+                t0[q] = k < nred ? acquire_f64(&redpart[3 * k + 0]) : 0.0;
+                t1[q] = k < nred ? acquire_f64(&redpart[3 * k + 1]) : 0.0;
+                t2[q] = k < nred ? acquire_f64(&redpart[3 * k + 2]) : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < kU; ++q) {
+                if (k0 + q * kWave < nred) {
+                    d0 += t0[q];
+                    d1 += t1[q];
+                    d2 += t2[q];
+                }
+            }
+        }
+    } else {
+        for (int k = lane; k < nred; k += kWave) {
+            d0 += redpart[3 * k + 0];
+            d1 += redpart[3 * k + 1];
+            d2 += redpart[3 * k + 2];
+        }
     }
     d0 = wave_sum(d0);
     d1 = wave_sum(d1);

@@ -20,13 +20,13 @@
 //   reduce_kernel  finishes the row/column sums (fixed order), applies the
 //                  dual update, the averaging of u, v, r, c and the vector
 //                  parts of interaction and movement.
-//   controller_kernel  one wave: the controller (the adaptive step-size
-//                  recurrence, the replay of a rejected step and the commit)
-//                  over the reduce blocks' scalars; commit_kernel does the
-//                  same for a pass left pending by an evaluation.  The kernel
-//                  boundary publishes the reduce blocks' results: no
-//                  device-scope fence (an L2 write-back and invalidate on
-//                  this GPU) runs per block.
+//                  Its last block to finish runs the controller (the
+//                  adaptive step-size recurrence, the replay of a rejected
+//                  step and the commit) over the blocks' scalars in one
+//                  wave; the blocks publish them with agent-scope stores and
+//                  an agent-scope completion count, with no device-scope
+//                  fence (an L2 write-back and invalidate on this GPU).
+//   commit_kernel  one wave: commits a pass left pending by an evaluation.
 // Evaluation of the point and its average (src/utils_gpu.jl:68-129) is fused
 // into the step kernel as extra reductions of the values it already loads.
 //
@@ -87,6 +87,8 @@ struct ReduceParams {
     const double* sqpart;
     double* redpart;
     double* evalvec;
+    uint32_t* done;           // blocks finished in this launch (0 between launches)
+    const int32_t* kind_err;  // see step_kernel
     int64_t n, m, ld;
     int32_t Cb, R, NBr, NBc, nred, nstep, sqchunk;
 };
@@ -179,14 +181,23 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
 // ----------------------------------------------------------- reduce kernel
 
 // Block b < NBr owns rows [64b, 64b+64), the next NBc blocks the columns;
-// reduce_chunk does the work; each block leaves its three scalars in
-// redpart for controller_kernel.
+// reduce_chunk does the work; each block publishes its three scalars in
+// redpart, and the last block to finish (an agent-scope completion count,
+// no device-scope fence) runs the controller over them in one wave: it
+// commits the pass, or leaves an evaluation's pass pending for the host's
+// decision.  Every block reads the state before it counts itself done, so
+// the last block may update it in place.  A halted solve (numerical error)
+// counts nothing and commits nothing.
 template <bool EVAL>
 __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     __shared__ pdot_solver_state sst;
     __shared__ double psum[kRedWaves][kWave];
+    __shared__ int32_t kind_err;
     const int tid = threadIdx.x;
-    if (tid == 0) sst = *p.st;
+    if (tid == 0) {
+        sst = *p.st;
+        kind_err = *p.kind_err;  // written by the step kernel before this one
+    }
     __syncthreads();
     const bool do_step = !sst.numerical_error;
     if (!do_step && !EVAL) return;
@@ -232,16 +243,36 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     c.paw = sst.pend_aw;
     double out3[3], ev[kVecStats];
     reduce_chunk<EVAL, kRedWaves>(c, psum, out3, ev);
+    __shared__ int last;
     if (tid == 0) {
         if (EVAL) {
             for (int t = 0; t < kVecStats; ++t) p.evalvec[(int64_t)blockIdx.x * kVecStats + t] = ev[t];
         }
+        last = 0;
         if (do_step) {
-            p.redpart[3 * blockIdx.x + 0] = out3[0];
-            p.redpart[3 * blockIdx.x + 1] = out3[1];
-            p.redpart[3 * blockIdx.x + 2] = out3[2];
+            publish_f64(&p.redpart[3 * blockIdx.x + 0], out3[0]);
+            publish_f64(&p.redpart[3 * blockIdx.x + 1], out3[1]);
+            publish_f64(&p.redpart[3 * blockIdx.x + 2], out3[2]);
+            wait_memory_ops();
+            last = count_done(p.done, gridDim.x) ? 1 : 0;
         }
     }
+    __syncthreads();
+    if (!last || tid >= kWave) return;
+    // the controller, in wave 0 of the last block
+    if (EVAL) {  // an evaluation's pass waits for the host's decision
+        if (tid == 0) p.st->pending = 1;
+        return;
+    }
+    if (kind_err) {  // the step kernel did not perform the pass (see step_kernel)
+        if (tid == 0) {
+            p.st->numerical_error = 1;
+            p.st->error_call = sst.take_step_calls + 1;
+        }
+        return;
+    }
+    reduce_and_commit<true>(&sst, p.redpart, p.nred);
+    if (tid == 0) *p.st = sst;
 }
 
 // Final fixed-order reduction of the evaluation partials into 22 values.
@@ -269,31 +300,6 @@ __global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart,
         acc = is_max ? block_max(acc, red) : block_sum(acc, red);
         if (tid == 0) out[kMatStats + k] = acc;
     }
-}
-
-// Commits the pass the reduce kernel has reduced; an evaluation's pass is
-// left pending for the host's decision instead (one wave).  A halted solve
-// (numerical error) commits nothing.
-template <bool EVAL>
-__global__ __launch_bounds__(kWave) void controller_kernel(pdot_solver_state* st, const double* redpart, int32_t nred,
-                                                           const int32_t* kind_err) {
-    __shared__ pdot_solver_state sst;
-    if (threadIdx.x == 0) sst = *st;
-    __syncthreads();
-    if (sst.numerical_error) return;
-    if (EVAL) {
-        if (threadIdx.x == 0) st->pending = 1;
-        return;
-    }
-    if (*kind_err) {  // the step kernel did not perform the pass (see step_kernel)
-        if (threadIdx.x == 0) {
-            st->numerical_error = 1;
-            st->error_call = sst.take_step_calls + 1;
-        }
-        return;
-    }
-    reduce_and_commit(&sst, redpart, nred);
-    if (threadIdx.x == 0) *st = sst;
 }
 
 // Commits a pending pass (one wave).
@@ -435,6 +441,7 @@ struct pdot_ctx {
     bool defer = true;  // ordinary passes alternate deferring / applying the average's update
     bool pass_kernels = true;     // the passes that leave the average alone use their own kernel
     int32_t* kind_err = nullptr;  // device flag, see step_kernel
+    uint32_t* done = nullptr;     // reduce_kernel's completion count, see there
     std::map<int64_t, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> events;
     int64_t bytes = 0;
@@ -582,6 +589,8 @@ ReduceParams reduce_params(pdot_ctx* ctx) {
     p.sqpart = ctx->sqpart;
     p.redpart = ctx->redpart;
     p.evalvec = ctx->evalvec;
+    p.done = ctx->done;
+    p.kind_err = ctx->kind_err;
     p.n = ctx->n;
     p.m = ctx->m;
     p.ld = ctx->ld;
@@ -636,12 +645,6 @@ hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
     dim3 grid(ctx->nred), block(kRedBlock);
     if (eval) hipLaunchKernelGGL((reduce_kernel<true>), grid, block, 0, ctx->stream, p);
     else hipLaunchKernelGGL((reduce_kernel<false>), grid, block, 0, ctx->stream, p);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    if (eval) hipLaunchKernelGGL((controller_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred,
-                                 ctx->kind_err);
-    else hipLaunchKernelGGL((controller_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred,
-                            ctx->kind_err);
     return hipGetLastError();
 }
 
@@ -874,6 +877,8 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     if ((rc = dev_alloc(ctx, (void**)&ctx->stA, sizeof(pdot_solver_state)))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->kind_err, sizeof(int32_t)))) return rc;
     HIPCHK(ctx, hipMemsetAsync(ctx->kind_err, 0, sizeof(int32_t), ctx->stream));
+    if ((rc = dev_alloc(ctx, (void**)&ctx->done, sizeof(uint32_t)))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->done, 0, sizeof(uint32_t), ctx->stream));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_state, sizeof(pdot_solver_state), hipHostMallocDefault));
     HIPCHK(ctx, hipHostMalloc((void**)&ctx->h_eval, sizeof(double) * kEvalOut, hipHostMallocDefault));
     if ((rc = configure_kernel(ctx))) return rc;
@@ -887,7 +892,7 @@ void pdot_destroy(pdot_ctx* ctx) {
     free_graphs(ctx);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     free_partials(ctx);
-    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->stage};
+    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->done, ctx->stage};
     for (void* q : ps)
         if (q) hipFree(q);
     for (int s = 0; s < 4; ++s) {
