@@ -397,16 +397,16 @@ __device__ __forceinline__ bool count_done(uint32_t* counter, uint32_t total) {
 // Reduce the per-block scalars of the reduce kernel (dot, ||dy||^2, ||dX||^2
 // chunk) in a fixed order and commit.  Called by the 64 lanes of one wave;
 // lane 0 updates *sst.  PUBLISHED: the scalars were written with
-// publish_f64 in the same kernel and are read with acquire_f64.
-template <bool PUBLISHED = false>
+// publish_f64 in the same kernel and are read with acquire_f64, the loads
+// of up to KU blocks per lane before the first sum (one memory round trip
+// for up to 64 KU blocks, KU * 6 more registers).
+template <bool PUBLISHED = false, int KU = 1>
 __device__ inline void reduce_and_commit(pdot_solver_state* sst, const double* redpart, int nred) {
     const int lane = threadIdx.x & (kWave - 1);
     double d0 = 0.0, d1 = 0.0, d2 = 0.0;
     if constexpr (PUBLISHED) {
-        // the loads of up to kU blocks per lane are issued before the first
-        // sum (one memory round trip for up to 64 kU blocks); the sums run
-        // in the same order as below
-        constexpr int kU = 8;
+        // the sums run in the same order as below
+        constexpr int kU = KU;
         for (int k0 = lane; k0 < nred; k0 += kWave * kU) {
             double t0[kU], t1[kU], t2[kU];
 #pragma unroll

@@ -271,7 +271,7 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
         }
         return;
     }
-    reduce_and_commit<true>(&sst, p.redpart, p.nred);
+    reduce_and_commit<true, 8>(&sst, p.redpart, p.nred);
     if (tid == 0) *p.st = sst;
 }
 
