@@ -238,6 +238,15 @@ int pdot_set_defer(pdot_ctx* ctx, int32_t enable);
  * counterpart in the reference (a kernel-selection choice of this library). */
 int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable);
 
+/* Where the controller (take_step!'s acceptance test, step-size recurrence
+ * and commit) runs: mode 1 in the last block of the reduce kernel to finish
+ * (its blocks publish their scalars without a device-scope fence), 0 in a
+ * one-wave kernel after it, -1 (the default) by size (1 for problems of up
+ * to 256 reduce blocks, n + m <= 16384).  Results are identical either way.
+ * No counterpart in the reference (a kernel-selection choice of this
+ * library). */
+int pdot_set_controller_fold(pdot_ctx* ctx, int32_t mode);
+
 /* Launch geometry of the streaming step kernel: rows per block, column
  * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the
  * default choice (used by the launch-shape tuner); 0 keeps the default.

@@ -19,13 +19,18 @@
 //                  Delta = max(X - tau G, 0) - X (5 n*m passes in total).
 //   reduce_kernel  finishes the row/column sums (fixed order), applies the
 //                  dual update, the averaging of u, v, r, c and the vector
-//                  parts of interaction and movement.
-//                  Its last block to finish runs the controller (the
-//                  adaptive step-size recurrence, the replay of a rejected
-//                  step and the commit) over the blocks' scalars in one
-//                  wave; the blocks publish them with agent-scope stores and
-//                  an agent-scope completion count, with no device-scope
-//                  fence (an L2 write-back and invalidate on this GPU).
+//                  parts of interaction and movement.  Then the controller
+//                  (the adaptive step-size recurrence, the replay of a
+//                  rejected step and the commit) runs over the blocks'
+//                  scalars in one wave: for problems of up to
+//                  kFoldMaxBlocks reduce blocks in the reduce kernel's last
+//                  block to finish (the blocks publish their scalars with
+//                  agent-scope stores and an agent-scope completion count,
+//                  with no device-scope fence, an L2 write-back and
+//                  invalidate on this GPU), for larger ones in
+//                  controller_kernel after the kernel boundary (the
+//                  controller's registers would halve the reduce kernel's
+//                  resident blocks).
 //   commit_kernel  one wave: commits a pass left pending by an evaluation.
 // Evaluation of the point and its average (src/utils_gpu.jl:68-129) is fused
 // into the step kernel as extra reductions of the values it already loads.
@@ -180,15 +185,24 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
 
 // ----------------------------------------------------------- reduce kernel
 
+// Reduce blocks up to which the controller runs in the reduce kernel's last
+// block (n + m <= 16384).  Measured per iteration (host clock, one process):
+// 34.0 -> 20.7 us at 1000 x 1000, 66.1 -> 60.0 us at 4096 x 4096 (FP32
+// storage), 580 -> 584 us at 10000 x 10000 (FP64, 314 blocks), and the
+// reduce kernel 6.0 + 6.9 -> 18.8 us at 30000 x 30000 (938 blocks, which no
+// longer fit in one round at the folded kernel's 90 registers).
+constexpr int32_t kFoldMaxBlocks = 256;
+
 // Block b < NBr owns rows [64b, 64b+64), the next NBc blocks the columns;
-// reduce_chunk does the work; each block publishes its three scalars in
-// redpart, and the last block to finish (an agent-scope completion count,
-// no device-scope fence) runs the controller over them in one wave: it
-// commits the pass, or leaves an evaluation's pass pending for the host's
-// decision.  Every block reads the state before it counts itself done, so
-// the last block may update it in place.  A halted solve (numerical error)
-// counts nothing and commits nothing.
-template <bool EVAL>
+// reduce_chunk does the work; each block leaves its three scalars in
+// redpart.  FOLD: the blocks publish them (agent-scope stores, then an
+// agent-scope completion count, no device-scope fence), and the last block
+// to finish runs the controller over them in one wave: it commits the pass,
+// or leaves an evaluation's pass pending for the host's decision.  Every
+// block reads the state before it counts itself done, so the last block may
+// update it in place.  A halted solve (numerical error) counts nothing and
+// commits nothing.  Without FOLD, controller_kernel follows.
+template <bool EVAL, bool FOLD>
 __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
     __shared__ pdot_solver_state sst;
     __shared__ double psum[kRedWaves][kWave];
@@ -249,7 +263,12 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
             for (int t = 0; t < kVecStats; ++t) p.evalvec[(int64_t)blockIdx.x * kVecStats + t] = ev[t];
         }
         last = 0;
-        if (do_step) {
+        if (do_step && !FOLD) {
+            p.redpart[3 * blockIdx.x + 0] = out3[0];
+            p.redpart[3 * blockIdx.x + 1] = out3[1];
+            p.redpart[3 * blockIdx.x + 2] = out3[2];
+        }
+        if (do_step && FOLD) {
             publish_f64(&p.redpart[3 * blockIdx.x + 0], out3[0]);
             publish_f64(&p.redpart[3 * blockIdx.x + 1], out3[1]);
             publish_f64(&p.redpart[3 * blockIdx.x + 2], out3[2]);
@@ -257,6 +276,7 @@ __global__ __launch_bounds__(kRedBlock) void reduce_kernel(ReduceParams p) {
             last = count_done(p.done, gridDim.x) ? 1 : 0;
         }
     }
+    if (!FOLD) return;
     __syncthreads();
     if (!last || tid >= kWave) return;
     // the controller, in wave 0 of the last block
@@ -300,6 +320,31 @@ __global__ __launch_bounds__(kBlock) void evalfin_kernel(const double* evalpart,
         acc = is_max ? block_max(acc, red) : block_sum(acc, red);
         if (tid == 0) out[kMatStats + k] = acc;
     }
+}
+
+// Commits the pass the reduce kernel (without FOLD) has reduced; an
+// evaluation's pass is left pending for the host's decision instead (one
+// wave).  A halted solve (numerical error) commits nothing.
+template <bool EVAL>
+__global__ __launch_bounds__(kWave) void controller_kernel(pdot_solver_state* st, const double* redpart, int32_t nred,
+                                                           const int32_t* kind_err) {
+    __shared__ pdot_solver_state sst;
+    if (threadIdx.x == 0) sst = *st;
+    __syncthreads();
+    if (sst.numerical_error) return;
+    if (EVAL) {
+        if (threadIdx.x == 0) st->pending = 1;
+        return;
+    }
+    if (*kind_err) {  // the step kernel did not perform the pass (see step_kernel)
+        if (threadIdx.x == 0) {
+            st->numerical_error = 1;
+            st->error_call = sst.take_step_calls + 1;
+        }
+        return;
+    }
+    reduce_and_commit(&sst, redpart, nred);
+    if (threadIdx.x == 0) *st = sst;
 }
 
 // Commits a pending pass (one wave).
@@ -440,6 +485,7 @@ struct pdot_ctx {
     bool use_graph = false;
     bool defer = true;  // ordinary passes alternate deferring / applying the average's update
     bool pass_kernels = true;     // the passes that leave the average alone use their own kernel
+    int32_t fold_mode = -1;       // controller in the reduce kernel: -1 by size, 0 never, 1 always
     int32_t* kind_err = nullptr;  // device flag, see step_kernel
     uint32_t* done = nullptr;     // reduce_kernel's completion count, see there
     std::map<int64_t, hipGraphExec_t> graphs;
@@ -643,8 +689,20 @@ hipError_t launch_step(pdot_ctx* ctx, bool eval, bool leave = false) {
 hipError_t launch_reduce(pdot_ctx* ctx, bool eval) {
     ReduceParams p = reduce_params(ctx);
     dim3 grid(ctx->nred), block(kRedBlock);
-    if (eval) hipLaunchKernelGGL((reduce_kernel<true>), grid, block, 0, ctx->stream, p);
-    else hipLaunchKernelGGL((reduce_kernel<false>), grid, block, 0, ctx->stream, p);
+    const bool fold = ctx->fold_mode == 1 || (ctx->fold_mode < 0 && ctx->nred <= kFoldMaxBlocks);
+    if (fold) {
+        if (eval) hipLaunchKernelGGL((reduce_kernel<true, true>), grid, block, 0, ctx->stream, p);
+        else hipLaunchKernelGGL((reduce_kernel<false, true>), grid, block, 0, ctx->stream, p);
+        return hipGetLastError();
+    }
+    if (eval) hipLaunchKernelGGL((reduce_kernel<true, false>), grid, block, 0, ctx->stream, p);
+    else hipLaunchKernelGGL((reduce_kernel<false, false>), grid, block, 0, ctx->stream, p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    if (eval) hipLaunchKernelGGL((controller_kernel<true>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred,
+                                 ctx->kind_err);
+    else hipLaunchKernelGGL((controller_kernel<false>), dim3(1), dim3(kWave), 0, ctx->stream, ctx->stA, ctx->redpart, ctx->nred,
+                            ctx->kind_err);
     return hipGetLastError();
 }
 
@@ -1378,6 +1436,16 @@ int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pass_kernels = enable != 0;
+    free_graphs(ctx);  // captured sequences hold the kernels
+    return PDOT_OK;
+}
+
+int pdot_set_controller_fold(pdot_ctx* ctx, int32_t mode) {
+    if (!ctx) return PDOT_E_ARG;
+    if (mode < -1 || mode > 1) return fail(ctx, PDOT_E_ARG, "pdot_set_controller_fold: mode must be -1, 0 or 1");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->fold_mode = mode;
     free_graphs(ctx);  // captured sequences hold the kernels
     return PDOT_OK;
 }

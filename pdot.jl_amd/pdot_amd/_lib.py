@@ -112,6 +112,7 @@ SIGNATURES = {
     "pdot_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_pass_kernels": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_controller_fold": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
@@ -316,6 +317,11 @@ class DeviceSolver:
 
     def set_pass_kernels(self, enable: bool) -> None:
         self._check(self.lib.pdot_set_pass_kernels(self.h, int(bool(enable))), "pdot_set_pass_kernels")
+
+    def set_controller_fold(self, mode: int) -> None:
+        """The controller in the reduce kernel's last block (1), in a kernel of
+        its own (0) or by size (-1, the default); results are identical."""
+        self._check(self.lib.pdot_set_controller_fold(self.h, int(mode)), "pdot_set_controller_fold")
 
     def set_launch(self, col_blocks_hint: int = 0, interleave: bool = True) -> None:
         self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint), int(bool(interleave))), "pdot_set_launch")
