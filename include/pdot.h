@@ -247,6 +247,13 @@ int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable);
  * library). */
 int pdot_set_controller_fold(pdot_ctx* ctx, int32_t mode);
 
+/* How the step kernel streams the matrices: mode 0 with the nontemporal hint,
+ * 1 without it (the lines may stay in the 256 MB Infinity Cache for the next
+ * pass), -1 (the default) by size (1 while five n x m arrays of the storage
+ * type take at most 352 MiB).  Results are identical either way.  No
+ * counterpart in the reference (a data-movement choice of this library). */
+int pdot_set_stream_policy(pdot_ctx* ctx, int32_t mode);
+
 /* Launch geometry of the streaming step kernel: rows per block, column
  * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the
  * default choice (used by the launch-shape tuner); 0 keeps the default.

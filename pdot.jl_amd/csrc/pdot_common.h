@@ -453,25 +453,18 @@ template <> struct VecT<float> {
     using type = flt4;
 };
 
-#ifndef PDOT_NT
-#define PDOT_NT 1
-#endif
-#ifndef PDOT_NT_LOAD
-#define PDOT_NT_LOAD PDOT_NT
-#endif
-#ifndef PDOT_NT_STORE
-#define PDOT_NT_STORE PDOT_NT
-#endif
-// 16-byte loads/stores of the streamed matrices; with the nontemporal hint
-// (the matrices are touched once per pass).
-template <typename V>
+// 16-byte loads/stores of the streamed matrices.  NT: with the nontemporal
+// hint, for matrices touched once per pass and far larger than the caches;
+// without it the lines may stay in the caches (the 256 MB Infinity Cache)
+// for the next pass, which pays where a pass's matrices nearly fit there.
+template <bool NT, typename V>
 __device__ __forceinline__ V ld16(const V* p) {
-    if constexpr (PDOT_NT_LOAD) return __builtin_nontemporal_load(p);
+    if constexpr (NT) return __builtin_nontemporal_load(p);
     else return *p;
 }
-template <typename V>
+template <bool NT, typename V>
 __device__ __forceinline__ void st16(V v, V* p) {
-    if constexpr (PDOT_NT_STORE) __builtin_nontemporal_store(v, p);
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
     else *p = v;
 }
 
@@ -490,17 +483,15 @@ __device__ __forceinline__ void unpack<flt4, float>(const flt4& v, double* o) {
     o[3] = (double)v.w;
 }
 
-template <typename T>
-__device__ __forceinline__ void store_vec(T* p, const double* o);
-template <>
-__device__ __forceinline__ void store_vec<double>(double* p, const double* o) {
+template <bool NT>
+__device__ __forceinline__ void store_vec(double* p, const double* o) {
     dbl2 v = {o[0], o[1]};
-    st16(v, reinterpret_cast<dbl2*>(p));
+    st16<NT>(v, reinterpret_cast<dbl2*>(p));
 }
-template <>
-__device__ __forceinline__ void store_vec<float>(float* p, const double* o) {
+template <bool NT>
+__device__ __forceinline__ void store_vec(float* p, const double* o) {
     flt4 v = {(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
-    st16(v, reinterpret_cast<flt4*>(p));
+    st16<NT>(v, reinterpret_cast<flt4*>(p));
 }
 
 // ------------------------------------------------------------ step tile
@@ -604,8 +595,11 @@ struct TileArgs {
 // (LDS, every kChunk columns).  With EVAL the same loads also produce the
 // evaluation reductions of the input point and of its average.  Called by
 // all threads of the block; thread 0 receives ||Delta||^2 of the tile in
-// *dd_out and the kMatStats evaluation reductions in ev_out.
-template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE, int COST = kCostDense, int KIND = kKindAny>
+// *dd_out and the kMatStats evaluation reductions in ev_out.  NT: the
+// matrices are streamed with the nontemporal hint (the arithmetic and its
+// order do not depend on it).
+template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE, int COST = kCostDense, int KIND = kKindAny,
+          bool NT = true>
 __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)[kChunk], double* red, double* dd_out,
                                           double* ev_out) {
     static_assert(!(EVAL && KIND == kKindLeave), "an evaluation's pass always updates the average");
@@ -689,9 +683,9 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             }
             if (colok && lane_active) {
                 const int64_t idx = j * ld + i0;
-                B.xv[q] = ld16(reinterpret_cast<const V*>(Xc + idx));
-                if constexpr (COST == kCostDense) B.cv[q] = ld16(reinterpret_cast<const V*>(C + idx));
-                if (!skip_avg) B.av[q] = ld16(reinterpret_cast<const V*>(Xa + idx));
+                B.xv[q] = ld16<NT>(reinterpret_cast<const V*>(Xc + idx));
+                if constexpr (COST == kCostDense) B.cv[q] = ld16<NT>(reinterpret_cast<const V*>(C + idx));
+                if (!skip_avg) B.av[q] = ld16<NT>(reinterpret_cast<const V*>(Xa + idx));
             }
         }
     };
@@ -772,8 +766,8 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                     }
                 }
                 if (do_step) {
-                    store_vec<T>(Xn + idx, xn);
-                    if (!skip_avg) store_vec<T>(Xan + idx, xan);
+                    store_vec<NT>(Xn + idx, xn);
+                    if (!skip_avg) store_vec<NT>(Xan + idx, xan);
                 }
             }
             const double s = wave_sum(csl);

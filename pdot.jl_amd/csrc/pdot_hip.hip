@@ -106,8 +106,9 @@ struct ReduceParams {
 // KIND = kKindLeave: a pass the host knows to leave the average alone (the
 // passes of a launch sequence alternate deterministically); should the state
 // say otherwise, the pass does nothing and flags kind_err, and the controller
-// then stops the solve instead of committing.
-template <typename T, bool EVAL, int COST, int KIND = kKindAny>
+// then stops the solve instead of committing.  NT: stream with the
+// nontemporal hint (see kCacheableMaxBytes).
+template <typename T, bool EVAL, int COST, int KIND = kKindAny, bool NT = true>
 __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave)) void step_kernel(StepParams p) {
     using Shape = StepShape<T, COST != kCostDense>;
     using LShape = LeaveShape<T, COST != kCostDense>;
@@ -174,7 +175,7 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     a.pnw = sst.pend_nw;
     a.paw = sst.pend_aw;
     double dd = 0.0, ev[kMatStats];
-    step_tile<T, EVAL, UNROLL, WAVES, PIPE, COST, KIND>(a, colbuf, red, &dd, ev);
+    step_tile<T, EVAL, UNROLL, WAVES, PIPE, COST, KIND, NT>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
         if (do_step) p.sqpart[blockIdx.x] = dd;
         if (EVAL) {
@@ -486,6 +487,7 @@ struct pdot_ctx {
     bool defer = true;  // ordinary passes alternate deferring / applying the average's update
     bool pass_kernels = true;     // the passes that leave the average alone use their own kernel
     int32_t fold_mode = -1;       // controller in the reduce kernel: -1 by size, 0 never, 1 always
+    int32_t stream_policy = -1;   // step kernel streams: -1 by size, 0 nontemporal, 1 cacheable
     int32_t* kind_err = nullptr;  // device flag, see step_kernel
     uint32_t* done = nullptr;     // reduce_kernel's completion count, see there
     std::map<int64_t, hipGraphExec_t> graphs;
@@ -650,22 +652,29 @@ ReduceParams reduce_params(pdot_ctx* ctx) {
     return p;
 }
 
-template <typename T, int COST>
+template <typename T, int COST, bool NT>
 void launch_step_c(bool eval, bool leave, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
-    if (eval) hipLaunchKernelGGL((step_kernel<T, true, COST>), grid, block, 0, stream, p);
+    if (eval) hipLaunchKernelGGL((step_kernel<T, true, COST, kKindAny, NT>), grid, block, 0, stream, p);
     else if constexpr (LeaveShape<T, COST != kCostDense>::kOwnKernel) {
-        if (leave) hipLaunchKernelGGL((step_kernel<T, false, COST, kKindLeave>), grid, block, 0, stream, p);
-        else hipLaunchKernelGGL((step_kernel<T, false, COST>), grid, block, 0, stream, p);
+        if (leave) hipLaunchKernelGGL((step_kernel<T, false, COST, kKindLeave, NT>), grid, block, 0, stream, p);
+        else hipLaunchKernelGGL((step_kernel<T, false, COST, kKindAny, NT>), grid, block, 0, stream, p);
     } else {
-        hipLaunchKernelGGL((step_kernel<T, false, COST>), grid, block, 0, stream, p);
+        hipLaunchKernelGGL((step_kernel<T, false, COST, kKindAny, NT>), grid, block, 0, stream, p);
     }
 }
 
+template <typename T, bool NT>
+void launch_step_n(int cost, bool eval, bool leave, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
+    if (cost == kCostSqEuclid) launch_step_c<T, kCostSqEuclid, NT>(eval, leave, grid, block, stream, p);
+    else if (cost == kCostL1) launch_step_c<T, kCostL1, NT>(eval, leave, grid, block, stream, p);
+    else launch_step_c<T, kCostDense, NT>(eval, leave, grid, block, stream, p);
+}
+
 template <typename T>
-void launch_step_t(int cost, bool eval, bool leave, dim3 grid, dim3 block, hipStream_t stream, const StepParams& p) {
-    if (cost == kCostSqEuclid) launch_step_c<T, kCostSqEuclid>(eval, leave, grid, block, stream, p);
-    else if (cost == kCostL1) launch_step_c<T, kCostL1>(eval, leave, grid, block, stream, p);
-    else launch_step_c<T, kCostDense>(eval, leave, grid, block, stream, p);
+void launch_step_t(int cost, bool nt, bool eval, bool leave, dim3 grid, dim3 block, hipStream_t stream,
+                   const StepParams& p) {
+    if (nt) launch_step_n<T, true>(cost, eval, leave, grid, block, stream, p);
+    else launch_step_n<T, false>(cost, eval, leave, grid, block, stream, p);
 }
 
 // leave: the pass is known to leave the average alone (kKindLeave kernel
@@ -677,12 +686,30 @@ bool uses_leave_kernel(pdot_ctx* ctx, bool eval, bool leave) {
     return points ? LeaveShape<float, true>::kOwnKernel : LeaveShape<float, false>::kOwnKernel;
 }
 
+// Largest footprint of a pass's matrices (five n x m arrays at the storage
+// type, whether or not C is stored, so that the point-cloud and dense forms
+// choose alike) streamed without the nontemporal hint.  Below it the lines a
+// pass leaves in the 256 MB Infinity Cache serve part of the next pass.
+// Measured per iteration without the hint against with it (one process,
+// gaussian-shift instances): 160 MiB (2048^2 FP64) -14%, 172 MiB (3000^2
+// FP32) -15%, 209 MiB -16%, 250 MiB -9%, 320 MiB (4096^2 FP32) -4%, 344 MiB
+// (3000^2 FP64) -7%; 390 MiB (4500^2 FP32) +4%, 419 MiB +8%, 482-964 MiB
+// +8-11%; at 20-39 MiB (1000^2) within 2% either way.
+constexpr int64_t kCacheableMaxBytes = int64_t(352) << 20;
+
+bool streams_nontemporal(const pdot_ctx* ctx) {
+    if (ctx->stream_policy == 0) return true;
+    if (ctx->stream_policy == 1) return false;
+    return 5 * ctx->ld * ctx->m * (int64_t)ctx->esz > kCacheableMaxBytes;
+}
+
 hipError_t launch_step(pdot_ctx* ctx, bool eval, bool leave = false) {
     leave = uses_leave_kernel(ctx, eval, leave);
     StepParams p = step_params(ctx);
     dim3 grid(ctx->nstep), block(step_waves(ctx->storage, ctx->cost_kind) * kWave);
-    if (ctx->storage == PDOT_STORAGE_F64) launch_step_t<double>(ctx->cost_kind, eval, leave, grid, block, ctx->stream, p);
-    else launch_step_t<float>(ctx->cost_kind, eval, leave, grid, block, ctx->stream, p);
+    const bool nt = streams_nontemporal(ctx);
+    if (ctx->storage == PDOT_STORAGE_F64) launch_step_t<double>(ctx->cost_kind, nt, eval, leave, grid, block, ctx->stream, p);
+    else launch_step_t<float>(ctx->cost_kind, nt, eval, leave, grid, block, ctx->stream, p);
     return hipGetLastError();
 }
 
@@ -1436,6 +1463,16 @@ int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable) {
     HIPCHK(ctx, hipSetDevice(ctx->device));
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pass_kernels = enable != 0;
+    free_graphs(ctx);  // captured sequences hold the kernels
+    return PDOT_OK;
+}
+
+int pdot_set_stream_policy(pdot_ctx* ctx, int32_t mode) {
+    if (!ctx) return PDOT_E_ARG;
+    if (mode < -1 || mode > 1) return fail(ctx, PDOT_E_ARG, "pdot_set_stream_policy: mode must be -1, 0 or 1");
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->stream_policy = mode;
     free_graphs(ctx);  // captured sequences hold the kernels
     return PDOT_OK;
 }

@@ -113,6 +113,7 @@ SIGNATURES = {
     "pdot_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_pass_kernels": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_controller_fold": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_stream_policy": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_device_bytes": (ctypes.c_int64, [_ctx_p]),
@@ -322,6 +323,11 @@ class DeviceSolver:
         """The controller in the reduce kernel's last block (1), in a kernel of
         its own (0) or by size (-1, the default); results are identical."""
         self._check(self.lib.pdot_set_controller_fold(self.h, int(mode)), "pdot_set_controller_fold")
+
+    def set_stream_policy(self, mode: int) -> None:
+        """Step-kernel streams with the nontemporal hint (0), without it (1) or
+        by size (-1, the default); results are identical."""
+        self._check(self.lib.pdot_set_stream_policy(self.h, int(mode)), "pdot_set_stream_policy")
 
     def set_launch(self, col_blocks_hint: int = 0, interleave: bool = True) -> None:
         self._check(self.lib.pdot_set_launch(self.h, int(col_blocks_hint), int(bool(interleave))), "pdot_set_launch")
