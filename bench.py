@@ -340,7 +340,9 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
                           of the solve that found them is not counted);
       learned_start       learned.solve_warm: everything the learned state offers,
                           behind its KKT acceptance test and fallback.
-    Prediction time is included in the learned rows.  A row's solve is capped
+    Prediction time (and for learned_start the KKT evaluation of the predicted
+    duals) is included in the learned rows; every row's solves share one
+    uploaded problem, and the output copy is not counted.  A row's solve is capped
     at limit_factor times the zero start's iterations."""
     import numpy as np
     import pdot_amd as P
@@ -383,12 +385,16 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
         if w_cal is not None:
             run("network_duals", w_cal, duals=(u0, v0), extra_sec=t_pred, cap=cap)
             run("exact_duals", w_cal, duals=found[0], cap=cap)
-        dev.close()
-        t0 = time.time()
+        # the learned start on the same uploaded problem: its dual prediction, the
+        # KKT evaluation of the predicted duals and the solve, timed as the rows above
+        prof = P.SolveProfile()
         out, info = learned.solve_warm(P.default_parameters(1e-4, record_iteration_stats=False), p, inst.x, inst.y,
-                                       net, storage=_lib.STORAGE_F32)
+                                       net, storage=_lib.STORAGE_F32, solver=dev, profile=prof)
+        dev.close()
         ci = out.iteration_stats[-1].convergence_information[0]
-        row["learned_start"] = {"seconds": time.time() - t0, "iterations": out.iteration_count,
+        row["learned_start"] = {"seconds": info.predict_sec + info.probe_sec + prof.solve_sec,
+                                "predict_sec": info.predict_sec, "probe_sec": info.probe_sec,
+                                "iterations": out.iteration_count,
                                 "termination": out.termination_string, "objective": ci.primal_objective,
                                 "duals_used": info.used_duals, "initial_primal_weight": info.primal_weight,
                                 "fell_back": info.fell_back}

@@ -292,6 +292,7 @@ class WarmStartInfo:
     primal_weight: Optional[float]   # the initial primal weight of the returned solve (None: the caller's)
     fell_back: bool                  # the warm solve missed OPTIMAL and the plain solve was returned
     predict_sec: float
+    probe_sec: float = 0.0           # the KKT evaluation of the predicted duals
 
 
 def learned_weight(net: WarmStartNet, x, y, a, b, path: str = DEFAULT_WEIGHTS) -> Optional[float]:
@@ -303,14 +304,32 @@ def learned_weight(net: WarmStartNet, x, y, a, b, path: str = DEFAULT_WEIGHTS) -
     return load_initial_primal_weight(path)
 
 
+def first_evaluation_kkt(params, dev: _lib.DeviceSolver, u0, v0) -> float:
+    """Relative KKT error of the starting state (X = 0, duals (u0, v0)) at a
+    solve's first evaluation (src/solver_gpu.jl:529-537, utils_gpu.jl:68-129),
+    from one device evaluation: the same quantity optimize() reports in its
+    first IterationStats, without running or downloading a solve."""
+    from . import solver as S
+    from .solve_log import PointType
+    ot_cache = dev.cached_ot_info()
+    policy, re, ge, step_size, weight = S.initial_step_and_weight(params, dev.n, dev.m, ot_cache)
+    dev.set_start(u0, v0, policy, re, ge, step_size, weight)
+    cur, _, _ = dev.eval()
+    crit = params.termination_criteria
+    ci = S.convergence_information(cur, ot_cache, crit.eps_optimal_absolute / crit.eps_optimal_relative,
+                                   PointType.POINT_TYPE_AVERAGE_ITERATE)
+    return S.relative_kkt_error(ci)
+
+
 def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE_F64, fallback: bool = True,
-               primal_weight: Optional[float] = None, use_calibrated_weight: bool = True):
+               primal_weight: Optional[float] = None, use_calibrated_weight: bool = True,
+               solver: Optional[_lib.DeviceSolver] = None, profile=None):
     """optimize() from the learned starting state, with the plain zero-start
     solve as the fallback and the solver's own KKT check deciding:
 
-    1. the predicted duals are evaluated (the first evaluation of a solve
-       limited to one iteration); if their relative KKT error is not below the
-       zero start's, the solve starts from zero duals;
+    1. the predicted duals are evaluated (the relative KKT error of a solve's
+       first evaluation); if it is not below the zero start's, the solve
+       starts from zero duals;
     2. the solve starts with the learned initial primal weight (the weight
        head's per-instance prediction if training selected it, else the
        calibrated constant; `primal_weight` overrides it and
@@ -318,7 +337,9 @@ def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE
     3. a warm solve that does not reach OPTIMAL is replaced by the plain
        zero-start solve with the caller's parameters.
 
-    Returns (output, info: WarmStartInfo)."""
+    solver: a DeviceSolver with the problem already uploaded (reused, not
+    closed); otherwise one is created for the call.  profile: a SolveProfile
+    for the returned solve.  Returns (output, info: WarmStartInfo)."""
     from dataclasses import replace
     from . import solver as S
     from .solve_log import TerminationReason
@@ -329,17 +350,24 @@ def solve_warm(params, problem, x, y, net: WarmStartNet, *, storage=_lib.STORAGE
     pred = time.time() - t0
     warm_params = params if primal_weight is None else replace(params, primal_importance=float(primal_weight))
     n, m = problem.cost_matrix.shape
-    dev = _lib.DeviceSolver(n, m, storage=storage)
+    dev = solver
+    if dev is None:
+        dev = _lib.DeviceSolver(n, m, storage=storage)
+    elif (dev.n, dev.m) != (n, m):
+        raise ValueError("solver was created for another problem size")
     try:
-        dev.upload(problem.cost_matrix, problem.source_distribution, problem.target_distribution)
-        probe = replace(params, termination_criteria=replace(params.termination_criteria, iteration_limit=1),
-                        verbosity=0)
-        first = S.optimize(probe, problem, storage=storage, warm_start=(u0, v0), solver=dev)
-        warm_kkt = S.relative_kkt_error(first.iteration_stats[0].convergence_information[0])
+        if solver is None:
+            dev.upload(problem.cost_matrix, problem.source_distribution, problem.target_distribution)
+        t0 = time.time()
+        warm_kkt = first_evaluation_kkt(params, dev, u0, v0)
+        probe = time.time() - t0
         used = bool(warm_kkt < zero_start_relative_kkt(problem))
-        out = S.optimize(warm_params, problem, storage=storage, warm_start=(u0, v0) if used else None, solver=dev)
+        out = S.optimize(warm_params, problem, storage=storage, warm_start=(u0, v0) if used else None, solver=dev,
+                         profile=profile)
         if out.termination_reason == TerminationReason.TERMINATION_REASON_OPTIMAL or not fallback:
-            return out, WarmStartInfo(used, primal_weight, False, pred)
-        return S.optimize(params, problem, storage=storage, solver=dev), WarmStartInfo(False, None, True, pred)
+            return out, WarmStartInfo(used, primal_weight, False, pred, probe)
+        return (S.optimize(params, problem, storage=storage, solver=dev, profile=profile),
+                WarmStartInfo(False, None, True, pred, probe))
     finally:
-        dev.close()
+        if solver is None:
+            dev.close()

@@ -56,6 +56,31 @@ def test_network_predicts_duals_better_than_zero(net):
     assert max(ratios) < 0.5
 
 
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_acceptance_probe_equals_the_first_evaluation_of_a_solve(net, storage):
+    """solve_warm judges the predicted duals by first_evaluation_kkt (one
+    device evaluation); it must be exactly the relative KKT error that a
+    solve started from them reports at its first evaluation, and a shared
+    device context must give the same solve as a fresh one."""
+    from dataclasses import replace
+    inst = instances.gmm_dirichlet(384, seed=HELD_OUT[0])
+    p = inst.problem
+    u0, v0 = net.predict_duals(inst.x, inst.y, p.source_distribution, p.target_distribution)
+    params = P.default_parameters(1e-4)
+    one = replace(params, termination_criteria=replace(params.termination_criteria, iteration_limit=1))
+    first = P.optimize(one, p, storage=storage, warm_start=(u0, v0))
+    dev = _lib.DeviceSolver(384, 384, storage=storage)
+    dev.upload(p.cost_matrix, p.source_distribution, p.target_distribution)
+    kkt = learned.first_evaluation_kkt(params, dev, u0, v0)
+    assert kkt == P.relative_kkt_error(first.iteration_stats[0].convergence_information[0])
+    shared, info = learned.solve_warm(params, p, inst.x, inst.y, net, storage=storage, solver=dev)
+    dev.close()
+    fresh, _ = learned.solve_warm(params, p, inst.x, inst.y, net, storage=storage)
+    assert shared.iteration_count == fresh.iteration_count
+    assert np.array_equal(shared.primal_solution, fresh.primal_solution)
+    assert info.probe_sec > 0
+
+
 def test_warm_started_solve_is_checked_and_correct(net):
     tol = 1e-4
     for seed in HELD_OUT:
