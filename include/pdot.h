@@ -254,6 +254,24 @@ int pdot_set_controller_fold(pdot_ctx* ctx, int32_t mode);
  * counterpart in the reference (a data-movement choice of this library). */
 int pdot_set_stream_policy(pdot_ctx* ctx, int32_t mode);
 
+/* Zero maps (on by default): every iterate slot keeps one bit per 16 bytes of
+ * each column saying whether those entries are all +0.0; a pass neither
+ * reads such entries nor writes a new point's entries where a group of 64
+ * bytes holds only +0.0, so the iterates' zeros (most of an optimal
+ * transport plan) cost no matrix traffic.  Results are bit for bit those of
+ * dense passes.  enable = 0 streams every entry (for comparison); the slots
+ * are made dense first.  No counterpart in the reference (a data-movement
+ * choice of this library). */
+int pdot_set_sparse(pdot_ctx* ctx, int32_t enable);
+
+/* Share of the point's 16-byte column units (`which`: PDOT_POINT_CURRENT or
+ * PDOT_POINT_AVERAGE, whose lagging contribution is written out first) that
+ * its zero map records as +0.0, i.e. that passes neither read nor write; 0
+ * when zero maps are off.  A unit is recorded as +0.0 only when it is; the
+ * share is exact for a point written by a pass with zero maps on.  No
+ * counterpart in the reference (a diagnostic of this library). */
+int pdot_zero_share(pdot_ctx* ctx, int32_t which, double* share);
+
 /* Launch geometry of the streaming step kernel: rows per block, column
  * blocks, columns per block, total blocks.  col_blocks_hint > 0 overrides the
  * default choice (used by the launch-shape tuner); 0 keeps the default.

@@ -28,25 +28,26 @@ constexpr int kMatStats = 10;     // per-block matrix eval reductions
 constexpr int kVecStats = 12;     // per-block vector eval reductions
 constexpr int kEvalOut = kMatStats + kVecStats;
 // Shape of the single-problem step kernel per storage type (measured best on
-// MI355X at 10k x 10k): waves per block (stacked along rows), columns per
-// load batch, and how many batches are prefetched ahead of the one computed.
+// MI355X over whole 10k x 10k solves with zero maps): waves per block
+// (stacked along rows), columns per load batch, and how many batches are
+// prefetched ahead of the one computed.
 #ifndef PDOT_F64_WAVES
 #define PDOT_F64_WAVES 8
 #endif
 #ifndef PDOT_F64_UNROLL
-#define PDOT_F64_UNROLL 1
+#define PDOT_F64_UNROLL 2
 #endif
 #ifndef PDOT_F64_PIPE
-#define PDOT_F64_PIPE 1
+#define PDOT_F64_PIPE 0
 #endif
 #ifndef PDOT_F32_WAVES
-#define PDOT_F32_WAVES 4
+#define PDOT_F32_WAVES 8
 #endif
 #ifndef PDOT_F32_UNROLL
 #define PDOT_F32_UNROLL 2
 #endif
 #ifndef PDOT_F32_PIPE
-#define PDOT_F32_PIPE 1
+#define PDOT_F32_PIPE 0
 #endif
 #ifndef PDOT_F64P_WAVES
 #define PDOT_F64P_WAVES PDOT_F64_WAVES
@@ -494,6 +495,33 @@ __device__ __forceinline__ void store_vec(float* p, const double* o) {
     st16<NT>(v, reinterpret_cast<flt4*>(p));
 }
 
+// Lanes per group whose 16-byte stores a sparse pass makes or skips
+// together (kZGroup * 16 = 64 bytes of a column; quad_any below).
+constexpr int kZGroup = 4;
+
+// True unless every value is +0.0 (a -0.0 counts as a value).
+template <int N>
+__device__ __forceinline__ bool any_bits(const double* o) {
+    uint64_t b = 0;
+#pragma unroll
+    for (int e = 0; e < N; ++e) b |= (uint64_t)__double_as_longlong(o[e]);
+    return b != 0;
+}
+
+// Lane `lane`'s bit of a wave-uniform word.
+__device__ __forceinline__ bool lane_bit(uint64_t word, int lane) {
+    const uint32_t half = lane < 32 ? (uint32_t)word : (uint32_t)(word >> 32);
+    return (half >> (lane & 31)) & 1;
+}
+
+// Whether any lane of this lane's group of four has x set (all lanes active).
+__device__ __forceinline__ bool quad_any(bool x) {
+    int v = x ? 1 : 0;
+    v |= __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    v |= __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    return v != 0;
+}
+
 // ------------------------------------------------------------ step tile
 
 // Where the cost C_ij comes from: the stored matrix, or formed in registers
@@ -586,6 +614,19 @@ struct TileArgs {
     // the storage type as a store would), then its own.
     bool defer, pend;
     double pnw, paw;
+    // Zero maps (sparse passes): one 64-bit word per wave and tile column,
+    // bit L for lane L's VEC rows.  A clear bit says those rows of that
+    // column are +0.0 in that slot, whatever its memory holds: they are not
+    // read, and a new point's rows are stored only where a group of
+    // kZGroup lanes holds a value other than +0.0.  Wave w's word for tile
+    // column t is at z* + w * zstride + t.
+    bool sparse;
+    const uint64_t* zc;  // current point's map
+    const uint64_t* za;  // average's map
+    uint64_t* zn;        // new point's map
+    uint64_t* zan;       // new average's map
+    int64_t zstride;
+    const T* zpage;      // 16 bytes of +0.0
 };
 
 // One PDHG pass over a tile (src/solver_gpu.jl:125-174 and 237-264,
@@ -597,9 +638,9 @@ struct TileArgs {
 // all threads of the block; thread 0 receives ||Delta||^2 of the tile in
 // *dd_out and the kMatStats evaluation reductions in ev_out.  NT: the
 // matrices are streamed with the nontemporal hint (the arithmetic and its
-// order do not depend on it).
+// order do not depend on it).  ZMAP: the tile may use zero maps (a.sparse).
 template <typename T, bool EVAL, int UNROLL, int WAVES, int PIPE, int COST = kCostDense, int KIND = kKindAny,
-          bool NT = true>
+          bool NT = true, bool ZMAP = true>
 __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)[kChunk], double* red, double* dd_out,
                                           double* ev_out) {
     static_assert(!(EVAL && KIND == kKindLeave), "an evaluation's pass always updates the average");
@@ -632,7 +673,28 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const bool gzero = a.gzero;
     const int64_t j0 = a.j0, jstep = a.jstep, ncols = a.ncols;
     const double crcp = COST != kCostDense ? 1.0 / a.cdiv : 1.0;  // point costs: the reciprocal of the scale
-
+    // zero maps: the words of a batch's columns are read (scalar loads, into
+    // zwc / zwa) while the batch before it is loaded, so they have arrived
+    // by the time its loads need them.  All bits set: read every lane.
+    const bool sparse = ZMAP && a.sparse;
+    const int64_t zoff = ZMAP ? (int64_t)__builtin_amdgcn_readfirstlane(w) * a.zstride : 0;
+    typedef const __attribute__((address_space(4))) uint64_t* ZPtr;
+    const ZPtr zc = (ZPtr)(a.zc + zoff);
+    const ZPtr za = (ZPtr)(a.za + zoff);
+    uint64_t* __restrict__ zn = a.zn + zoff;
+    uint64_t* __restrict__ zan = a.zan + zoff;
+    uint64_t zwc[UNROLL], zwa[UNROLL];
+    auto load_zwords = [&](int64_t tb) {  // block-uniform
+#pragma unroll
+        for (int q = 0; q < UNROLL; ++q) {
+            zwc[q] = ~0ull;
+            zwa[q] = ~0ull;
+            if (sparse && tb + q < ncols) {
+                zwc[q] = zc[tb + q];
+                if (!skip_avg) zwa[q] = za[tb + q];
+            }
+        }
+    };
     double ui[VEC], uai[VEC];
     bool valid[VEC];
 #pragma unroll
@@ -681,13 +743,20 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
 #pragma unroll
                 for (int d = 0; d < kMaxDim; ++d) B.yj[q][d] = (colok && d < a.dim) ? a.py[j * a.dim + d] : 0.0;
             }
+            const uint64_t wc = zwc[q], wa = zwa[q];
             if (colok && lane_active) {
                 const int64_t idx = j * ld + i0;
-                B.xv[q] = ld16<NT>(reinterpret_cast<const V*>(Xc + idx));
+                // a lane whose rows are +0.0 loads the zero page instead
+                const T* xp = lane_bit(wc, lane) ? Xc + idx : a.zpage;
+                B.xv[q] = ld16<NT>(reinterpret_cast<const V*>(xp));
                 if constexpr (COST == kCostDense) B.cv[q] = ld16<NT>(reinterpret_cast<const V*>(C + idx));
-                if (!skip_avg) B.av[q] = ld16<NT>(reinterpret_cast<const V*>(Xa + idx));
+                if (!skip_avg) {
+                    const T* ap = lane_bit(wa, lane) ? Xa + idx : a.zpage;
+                    B.av[q] = ld16<NT>(reinterpret_cast<const V*>(ap));
+                }
             }
         }
+        load_zwords(tb + UNROLL);  // the next batch to be loaded
     };
     auto compute_batch = [&](int64_t tb, const Batch& B) {
         const int64_t tc = tb / kChunk * kChunk;
@@ -696,9 +765,10 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             const bool colok = tb + q < ncols;
             const int64_t j = j0 + (tb + q) * jstep;
             double csl = 0.0;
+            double xn[VEC], xan[VEC];
+            bool live = false, alive = false;
             if (colok && lane_active) {
-                const int64_t idx = j * ld + i0;
-                double x[VEC], c[VEC], xa[VEC], xn[VEC], xan[VEC];
+                double x[VEC], c[VEC], xa[VEC];
                 unpack<V, T>(B.xv[q], x);
                 if constexpr (COST == kCostDense) {
                     unpack<V, T>(B.cv[q], c);
@@ -765,9 +835,23 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                         xan[e] = t * aw;
                     }
                 }
-                if (do_step) {
-                    store_vec<NT>(Xn + idx, xn);
-                    if (!skip_avg) store_vec<NT>(Xan + idx, xan);
+                live = any_bits<VEC>(xn);
+                alive = !skip_avg && any_bits<VEC>(xan);
+            }
+            // block-uniform: all lanes take part in the ballots
+            const uint64_t bn = sparse ? __builtin_amdgcn_ballot_w64(live) : ~0ull;
+            const uint64_t ban = sparse ? __builtin_amdgcn_ballot_w64(alive) : ~0ull;
+            const bool gn = !sparse || quad_any(live);
+            const bool gan = !sparse || quad_any(alive);
+            if (do_step && colok) {
+                if (lane_active) {
+                    const int64_t idx = j * ld + i0;
+                    if (gn) store_vec<NT>(Xn + idx, xn);
+                    if (!skip_avg && gan) store_vec<NT>(Xan + idx, xan);
+                }
+                if (sparse && lane == 0) {
+                    zn[tb + q] = bn;
+                    if (!skip_avg) zan[tb + q] = ban;
                 }
             }
             const double s = wave_sum(csl);
@@ -787,6 +871,7 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
         }
     };
     static_assert(kChunk % UNROLL == 0, "a batch must not straddle a column-sum flush");
+    load_zwords(0);
     Batch ring[PIPE + 1];
 #pragma unroll
     for (int k = 0; k < PIPE; ++k)

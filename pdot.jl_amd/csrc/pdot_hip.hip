@@ -80,6 +80,10 @@ struct StepParams {
     int32_t TRB;         // rows per row block: n split evenly over the R blocks (<= the block's rows)
     int32_t interleave;  // 1: block cb takes columns cb, cb+Cb, ...; 0: a contiguous range
     int32_t* kind_err;   // set by a leave-alone pass whose state says it must update the average
+    uint64_t* zmap[4];   // zero map of each X slot (see ZGeom)
+    int64_t ztc;         // map words per wave and tile: the most columns a tile has
+    int32_t sparse;      // passes read and write through the zero maps
+    const void* zpage;   // 16 bytes of +0.0
 };
 
 struct ReduceParams {
@@ -174,6 +178,14 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     a.pend = sst.avg_pending != 0;
     a.pnw = sst.pend_nw;
     a.paw = sst.pend_aw;
+    a.sparse = p.sparse != 0;
+    const int64_t zbase = ((int64_t)rb * p.Cb + cb) * WAVES * p.ztc;
+    a.zc = p.zmap[cs] + zbase;
+    a.za = p.zmap[as] + zbase;
+    a.zn = p.zmap[sst.spare0] + zbase;
+    a.zan = p.zmap[sst.spare1] + zbase;
+    a.zstride = p.ztc;
+    a.zpage = static_cast<const T*>(p.zpage);
     double dd = 0.0, ev[kMatStats];
     step_tile<T, EVAL, UNROLL, WAVES, PIPE, COST, KIND, NT>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
@@ -440,6 +452,76 @@ __global__ void widen_kernel(const T* __restrict__ src, double* __restrict__ dst
     }
 }
 
+// Geometry of the zero maps: word ((rb * Cb + cb) * waves + w) * tc + t
+// covers, in column j of tile column t (j = cb + t Cb interleaved, else
+// cb TC + t), the vec rows row0 + (w * 64 + L) * vec of lane L, row0 =
+// rb * TRB, for the lanes below the row block's end.
+struct ZGeom {
+    int64_t n = 0, m = 0;
+    int64_t ld = 0;
+    int32_t R = 0, Cb = 0, TRB = 0, TC = 0, interleave = 0, waves = 0, vec = 0;
+    int64_t tc = 0;     // words per wave and tile
+    int64_t words = 0;  // R * Cb * waves * tc
+};
+
+// Writes +0.0 into every lane whose map bit is clear, so that the slot's
+// memory holds every value (the map stays valid).  One wave per word.
+template <typename T>
+__global__ __launch_bounds__(256) void densify_kernel(T* __restrict__ X, const uint64_t* __restrict__ z, ZGeom g) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
+    for (int64_t wi = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < g.words; wi += nw) {
+        const int64_t t = wi % g.tc;
+        int64_t r = wi / g.tc;
+        const int64_t w = r % g.waves;
+        r /= g.waves;
+        const int64_t cb = r % g.Cb, rb = r / g.Cb;
+        int64_t ncols, j;
+        if (g.interleave) {
+            ncols = (g.m - cb + g.Cb - 1) / g.Cb;
+            j = cb + t * g.Cb;
+        } else {
+            const int64_t j0 = cb * g.TC;
+            ncols = ((j0 + g.TC < g.m) ? j0 + g.TC : g.m) - j0;
+            j = j0 + t;
+        }
+        const int64_t row0 = rb * g.TRB;
+        const int64_t row_end = (row0 + g.TRB < g.n) ? row0 + g.TRB : g.n;
+        const int64_t i0 = row0 + (w * kWave + lane) * g.vec;
+        if (t >= ncols || i0 >= row_end || ((z[wi] >> lane) & 1)) continue;
+        for (int e = 0; e < g.vec; ++e) X[j * g.ld + i0 + e] = (T)0;
+    }
+}
+
+// Lanes marked as holding a value (set bits of the lanes below each row
+// block's end), summed into *out.
+__global__ __launch_bounds__(256) void zcount_kernel(const uint64_t* __restrict__ z, ZGeom g,
+                                                     unsigned long long* __restrict__ out) {
+    unsigned long long live = 0;
+    for (int64_t wi = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; wi < g.words; wi += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t t = wi % g.tc;
+        int64_t r = wi / g.tc;
+        const int64_t w = r % g.waves;
+        r /= g.waves;
+        const int64_t cb = r % g.Cb, rb = r / g.Cb;
+        const int64_t ncols = g.interleave ? (g.m - cb + g.Cb - 1) / g.Cb
+                                           : ((cb * g.TC + g.TC < g.m) ? cb * g.TC + g.TC : g.m) - cb * g.TC;
+        const int64_t row0 = rb * g.TRB + w * kWave * g.vec;
+        const int64_t row_end = (rb * g.TRB + g.TRB < g.n) ? rb * g.TRB + g.TRB : g.n;
+        if (t >= ncols || row0 >= row_end) continue;
+        const int64_t lanes = (row_end - row0 + g.vec - 1) / g.vec;
+        const uint64_t mask = lanes >= 64 ? ~0ull : ((1ull << lanes) - 1);
+        live += (unsigned long long)__popcll(z[wi] & mask);
+    }
+    for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
+    if ((threadIdx.x & (kWave - 1)) == 0 && live) atomicAdd(out, live);
+}
+
+__global__ __launch_bounds__(256) void fill_u64_kernel(uint64_t* __restrict__ z, int64_t count, uint64_t value) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        z[i] = value;
+}
+
 }  // namespace
 
 // ====================================================================== host
@@ -488,6 +570,10 @@ struct pdot_ctx {
     bool pass_kernels = true;     // the passes that leave the average alone use their own kernel
     int32_t fold_mode = -1;       // controller in the reduce kernel: -1 by size, 0 never, 1 always
     int32_t stream_policy = -1;   // step kernel streams: -1 by size, 0 nontemporal, 1 cacheable
+    bool sparse = true;           // passes skip the zero rows recorded in the zero maps
+    uint64_t* zmap[4] = {nullptr, nullptr, nullptr, nullptr};  // per X slot (geometry zg)
+    ZGeom zg;
+    void* zpage = nullptr;        // 256 bytes of +0.0: what the lanes of known zeros load
     int32_t* kind_err = nullptr;  // device flag, see step_kernel
     uint32_t* done = nullptr;     // reduce_kernel's completion count, see there
     std::map<int64_t, hipGraphExec_t> graphs;
@@ -548,10 +634,82 @@ void free_partials(pdot_ctx* ctx) {
     }
 }
 
+// Zero maps.  Makes slot `slot`'s memory hold every value (its map stays valid).
+int densify_slot(pdot_ctx* ctx, int slot) {
+    if (!ctx->zmap[slot] || ctx->zg.words == 0) return PDOT_OK;
+    const ZGeom& g = ctx->zg;
+    const int64_t blocks = std::min<int64_t>((g.words + 3) / 4, 8192);
+    if (ctx->storage == PDOT_STORAGE_F64)
+        hipLaunchKernelGGL(densify_kernel<double>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           static_cast<double*>(ctx->s.X[slot]), ctx->zmap[slot], g);
+    else
+        hipLaunchKernelGGL(densify_kernel<float>, dim3(blocks), dim3(256), 0, ctx->stream,
+                           static_cast<float*>(ctx->s.X[slot]), ctx->zmap[slot], g);
+    HIPCHK(ctx, hipGetLastError());
+    return PDOT_OK;
+}
+
+int fill_map(pdot_ctx* ctx, int slot, uint64_t value) {
+    if (!ctx->zmap[slot] || ctx->zg.words == 0) return PDOT_OK;
+    const int64_t blocks = std::min<int64_t>((ctx->zg.words + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(blocks), dim3(256), 0, ctx->stream, ctx->zmap[slot], ctx->zg.words, value);
+    HIPCHK(ctx, hipGetLastError());
+    return PDOT_OK;
+}
+
+// Before the maps are dropped or no longer kept up to date: every slot's
+// memory holds every value.
+int densify_all(pdot_ctx* ctx) {
+    int rc;
+    for (int s = 0; s < 4; ++s)
+        if ((rc = densify_slot(ctx, s))) return rc;
+    return PDOT_OK;
+}
+
+void free_maps(pdot_ctx* ctx) {
+    for (int s = 0; s < 4; ++s) {
+        if (ctx->zmap[s]) {
+            hipFree(ctx->zmap[s]);
+            ctx->bytes -= (int64_t)(sizeof(uint64_t) * (size_t)ctx->zg.words);
+        }
+        ctx->zmap[s] = nullptr;
+    }
+    ctx->zg = ZGeom();
+}
+
+// Maps for the present geometry, all bits set (the slots' memory holds every value).
+int alloc_maps(pdot_ctx* ctx) {
+    ZGeom g;
+    g.n = ctx->n;
+    g.m = ctx->m;
+    g.ld = ctx->ld;
+    g.R = ctx->R;
+    g.Cb = ctx->Cb;
+    g.TRB = ctx->TRB;
+    g.TC = ctx->TC;
+    g.interleave = ctx->interleave;
+    g.waves = step_waves(ctx->storage, ctx->cost_kind);
+    g.vec = (int32_t)(16 / ctx->esz);
+    g.tc = ctx->interleave ? (ctx->m + ctx->Cb - 1) / ctx->Cb : ctx->TC;
+    g.words = (int64_t)g.R * g.Cb * g.waves * g.tc;
+    int rc;
+    for (int s = 0; s < 4; ++s)
+        if ((rc = dev_alloc(ctx, (void**)&ctx->zmap[s], sizeof(uint64_t) * (size_t)g.words))) return rc;
+    ctx->zg = g;
+    for (int s = 0; s < 4; ++s)
+        if ((rc = fill_map(ctx, s, ~0ull))) return rc;
+    return PDOT_OK;
+}
+
 // Launch geometry: R row blocks of rows_per_block rows; Cb column blocks.
 int setup_geometry(pdot_ctx* ctx) {
     free_graphs(ctx);
     free_partials(ctx);  // evalvec, allocated last, stays null if an allocation below fails
+    // the maps describe the slots in the geometry being replaced
+    int rc0 = ctx->sparse ? densify_all(ctx) : PDOT_OK;
+    if (rc0) return rc0;
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    free_maps(ctx);
     const int TR = rows_per_block(ctx->storage, ctx->cost_kind);
     const int64_t R = (ctx->n + TR - 1) / TR;
     // One full wave of resident blocks: every block streams the same number
@@ -597,6 +755,7 @@ int setup_geometry(pdot_ctx* ctx) {
     if ((rc = dev_alloc(ctx, (void**)&ctx->sqpart, sizeof(double) * (size_t)ctx->nstep))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalpart, sizeof(double) * (size_t)ctx->nstep * kMatStats))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->redpart, sizeof(double) * (size_t)ctx->nred * 3))) return rc;
+    if ((rc = alloc_maps(ctx))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalvec, sizeof(double) * (size_t)ctx->nred * kVecStats))) return rc;
     return PDOT_OK;
 }
@@ -623,6 +782,10 @@ StepParams step_params(pdot_ctx* ctx) {
     p.R = ctx->R;
     p.interleave = ctx->interleave;
     p.kind_err = ctx->kind_err;
+    for (int k = 0; k < 4; ++k) p.zmap[k] = ctx->zmap[k];
+    p.ztc = ctx->zg.tc;
+    p.sparse = (ctx->sparse && ctx->zg.words > 0) ? 1 : 0;
+    p.zpage = ctx->zpage;
     return p;
 }
 
@@ -812,6 +975,12 @@ int write_state(pdot_ctx* ctx, const pdot_solver_state* in) {
 int materialize_average(pdot_ctx* ctx, pdot_solver_state& st) {
     if (!st.avg_pending) return PDOT_OK;
     const int64_t total = ctx->ld * ctx->m;
+    int rc;
+    if (ctx->sparse) {  // the kernel below reads and writes every value
+        if ((rc = densify_slot(ctx, st.avg_slot))) return rc;
+        if ((rc = densify_slot(ctx, st.cur_slot))) return rc;
+        if ((rc = fill_map(ctx, st.avg_slot, ~0ull))) return rc;
+    }
     if (ctx->storage == PDOT_STORAGE_F64)
         hipLaunchKernelGGL(materialize_kernel<double>, dim3(4096), dim3(256), 0, ctx->stream,
                            static_cast<double*>(ctx->s.X[st.avg_slot]), static_cast<const double*>(ctx->s.X[st.cur_slot]),
@@ -960,6 +1129,8 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     if ((rc = dev_alloc(ctx, (void**)&ctx->b, sizeof(double) * (size_t)m))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalout, sizeof(double) * kEvalOut))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->stA, sizeof(pdot_solver_state)))) return rc;
+    if ((rc = dev_alloc(ctx, &ctx->zpage, 256))) return rc;
+    HIPCHK(ctx, hipMemsetAsync(ctx->zpage, 0, 256, ctx->stream));
     if ((rc = dev_alloc(ctx, (void**)&ctx->kind_err, sizeof(int32_t)))) return rc;
     HIPCHK(ctx, hipMemsetAsync(ctx->kind_err, 0, sizeof(int32_t), ctx->stream));
     if ((rc = dev_alloc(ctx, (void**)&ctx->done, sizeof(uint32_t)))) return rc;
@@ -977,7 +1148,8 @@ void pdot_destroy(pdot_ctx* ctx) {
     free_graphs(ctx);
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     free_partials(ctx);
-    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->done, ctx->stage};
+    free_maps(ctx);
+    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->done, ctx->stage, ctx->zpage};
     for (void* q : ps)
         if (q) hipFree(q);
     for (int s = 0; s < 4; ++s) {
@@ -1165,8 +1337,10 @@ int pdot_set_start(pdot_ctx* ctx, const double* u0, const double* v0, int32_t st
         return fail(ctx, PDOT_E_ARG, "pdot_set_start: unknown step policy");
     HIPCHK(ctx, hipSetDevice(ctx->device));
     const size_t mat = ctx->esz * (size_t)ctx->ld * (size_t)ctx->m;
+    int rc;
     for (int s = 0; s < 4; ++s) {
         HIPCHK(ctx, hipMemsetAsync(ctx->s.X[s], 0, mat, ctx->stream));
+        if ((rc = fill_map(ctx, s, 0))) return rc;  // every row +0.0
         HIPCHK(ctx, hipMemsetAsync(ctx->s.u[s], 0, sizeof(double) * ctx->ld, ctx->stream));
         HIPCHK(ctx, hipMemsetAsync(ctx->s.r[s], 0, sizeof(double) * ctx->ld, ctx->stream));
         HIPCHK(ctx, hipMemsetAsync(ctx->s.v[s], 0, sizeof(double) * ctx->m, ctx->stream));
@@ -1189,8 +1363,7 @@ int pdot_set_start(pdot_ctx* ctx, const double* u0, const double* v0, int32_t st
     st.spare0 = 1;
     st.spare1 = 2;
     st.defer_ok = ctx->defer ? 1 : 0;
-    int rc = write_state(ctx, &st);
-    if (rc) return rc;
+    if ((rc = write_state(ctx, &st))) return rc;
     ctx->started = true;
     return PDOT_OK;
 }
@@ -1345,6 +1518,7 @@ int pdot_download(pdot_ctx* ctx, int32_t which, double* X, double* u, double* v)
         if ((rc = materialize_average(ctx, st))) return rc;
         slot = st.avg_slot;
     } else return fail(ctx, PDOT_E_ARG, "pdot_download: unknown point");
+    if (X && ctx->sparse && (rc = densify_slot(ctx, slot))) return rc;
     const int64_t n = ctx->n, m = ctx->m, ld = ctx->ld;
     if (X) {
         if (ctx->storage == PDOT_STORAGE_F64) {
@@ -1380,6 +1554,7 @@ int pdot_round(pdot_ctx* ctx, int32_t which, double* X_rounded) {
         slot = st.avg_slot;
     } else return fail(ctx, PDOT_E_ARG, "pdot_round: unknown point");
     HIPCHK(ctx, hipSetDevice(ctx->device));
+    if (ctx->sparse && (rc = densify_slot(ctx, slot))) return rc;
     double* Y = nullptr;
     if ((rc = dev_alloc(ctx, (void**)&Y, sizeof(double) * (size_t)(ctx->n * ctx->m)))) return rc;
     ctx->bytes -= (int64_t)(sizeof(double) * (size_t)(ctx->n * ctx->m));  // released below
@@ -1474,6 +1649,61 @@ int pdot_set_stream_policy(pdot_ctx* ctx, int32_t mode) {
     HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
     ctx->stream_policy = mode;
     free_graphs(ctx);  // captured sequences hold the kernels
+    return PDOT_OK;
+}
+
+int pdot_set_sparse(pdot_ctx* ctx, int32_t enable) {
+    if (!ctx) return PDOT_E_ARG;
+    if (enable != 0 && enable != 1) return fail(ctx, PDOT_E_ARG, "pdot_set_sparse: enable must be 0 or 1");
+    if ((enable != 0) == ctx->sparse) return PDOT_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    int rc;
+    if (enable) {
+        // dense passes kept no maps: every bit set, then kept from here on
+        for (int s = 0; s < 4; ++s)
+            if ((rc = fill_map(ctx, s, ~0ull))) return rc;
+    } else {
+        if ((rc = densify_all(ctx))) return rc;
+    }
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    free_graphs(ctx);  // the captured passes carry the mode
+    ctx->sparse = enable != 0;
+    return PDOT_OK;
+}
+
+int pdot_zero_share(pdot_ctx* ctx, int32_t which, double* share) {
+    if (!ctx || !share) return PDOT_E_ARG;
+    if (!ctx->started) return fail(ctx, PDOT_E_STATE, "pdot_zero_share before pdot_set_start");
+    pdot_solver_state st;
+    int rc = read_state(ctx, &st);
+    if (rc) return rc;
+    int slot;
+    if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
+    else if (which == PDOT_POINT_AVERAGE) {
+        if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_zero_share: the average is empty");
+        if ((rc = materialize_average(ctx, st))) return rc;
+        slot = st.avg_slot;
+    } else return fail(ctx, PDOT_E_ARG, "pdot_zero_share: unknown point");
+    *share = 0.0;
+    if (!ctx->sparse || ctx->zg.words == 0) return PDOT_OK;
+    HIPCHK(ctx, hipSetDevice(ctx->device));
+    unsigned long long* d = nullptr;
+    if ((rc = dev_alloc(ctx, (void**)&d, sizeof(unsigned long long)))) return rc;
+    ctx->bytes -= (int64_t)sizeof(unsigned long long);  // released below
+    unsigned long long live = 0;
+    hipError_t e = hipMemsetAsync(d, 0, sizeof(live), ctx->stream);
+    if (e == hipSuccess) {
+        const int64_t blocks = std::min<int64_t>((ctx->zg.words + 255) / 256, 2048);
+        hipLaunchKernelGGL(zcount_kernel, dim3(blocks), dim3(256), 0, ctx->stream, ctx->zmap[slot], ctx->zg, d);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(&live, d, sizeof(live), hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    hipFree(d);
+    if (e != hipSuccess) return fail(ctx, PDOT_E_HIP, std::string("pdot_zero_share: ") + hipGetErrorString(e));
+    const int64_t vec = 16 / ctx->esz;
+    const double units = (double)((ctx->n + vec - 1) / vec) * (double)ctx->m;
+    *share = 1.0 - (double)live / units;
     return PDOT_OK;
 }
 

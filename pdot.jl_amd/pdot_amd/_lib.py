@@ -113,6 +113,8 @@ SIGNATURES = {
     "pdot_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_pass_kernels": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_controller_fold": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_set_sparse": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_zero_share": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "pdot_set_stream_policy": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
@@ -323,6 +325,17 @@ class DeviceSolver:
         """The controller in the reduce kernel's last block (1), in a kernel of
         its own (0) or by size (-1, the default); results are identical."""
         self._check(self.lib.pdot_set_controller_fold(self.h, int(mode)), "pdot_set_controller_fold")
+
+    def set_sparse(self, enable: bool) -> None:
+        """Zero maps on (default: entries known to be +0.0 are neither read
+        nor written) or off (every entry streamed); results are identical."""
+        self._check(self.lib.pdot_set_sparse(self.h, int(bool(enable))), "pdot_set_sparse")
+
+    def zero_share(self, which: int = POINT_CURRENT) -> float:
+        """Share of the point's 16-byte column units recorded as +0.0 (skipped by passes)."""
+        out = ctypes.c_double()
+        self._check(self.lib.pdot_zero_share(self.h, int(which), ctypes.byref(out)), "pdot_zero_share")
+        return out.value
 
     def set_stream_policy(self, mode: int) -> None:
         """Step-kernel streams with the nontemporal hint (0), without it (1) or
