@@ -9,6 +9,13 @@ average).  The timed region runs K steps of the FP64 form (the reference's
 precision) on inputs resident in HBM, with HIP events recorded on the
 library's stream around every launch of the streaming step kernel, so the
 same run gives the iteration rate and the step kernel's average duration.
+It starts --start-iteration (2048) iterations into the solve (untimed, with
+the solver's own evaluations and restarts) and W untimed passes later: the
+passes skip the iterates' entries recorded as +0.0 (zero maps), so their cost
+depends on how many there are, and the zero start has only zeros.  Of the
+points sampled along the 10k solve, iteration 2048 has the fewest.  The
+roofline counts the bytes the passes must move there (zero shares measured
+before and after the timed passes); the dense figure is reported beside it.
 
 Also reported (extra fields): the same measurement for the FP32-storage and
 point-cloud forms, full solves to 1e-4 relative KKT (time from after the
@@ -83,14 +90,37 @@ def measured_batch_traffic():
     return json.loads(text.split("```json", 1)[1].split("```", 1)[0])["traffic_bytes"]
 
 
-def measure_steps(dev, steps: int, warmup: int):
+def measure_steps(dev, steps: int, warmup: int, prob=None, storage=None, start: int = 0):
+    """Brings the device to `start` iterations into the solve of `prob` (the
+    reference's defaults, tolerance 1e-4: its evaluations and restarts
+    included, untimed), or to a fresh zero start; then `warmup` untimed and
+    `steps` timed passes.  The zero shares of the current point and of the
+    average's slot are read before and after the timed passes."""
+    import pdot_amd as P
     from pdot_amd import _lib
-    dev.set_start(policy=_lib.POLICY_ADAPTIVE, step_size=1.0, primal_weight=1.0)
+    reached = 0
+    if prob is not None and start > 0:
+        params = P.default_parameters(1e-4, iteration_limit=start, verbosity=0, record_iteration_stats=False)
+        reached = P.optimize(params, prob, storage=storage, solver=dev).iteration_count
+    else:
+        dev.set_start(policy=_lib.POLICY_ADAPTIVE, step_size=1.0, primal_weight=1.0)
     if warmup:
         dev.time_steps(warmup)
+
+    def shares():
+        zc = dev.zero_share(_lib.POINT_CURRENT)
+        try:
+            za = dev.zero_share(_lib.POINT_AVERAGE)
+        except _lib.PdotError:  # empty average
+            za = zc
+        return zc, za
+
+    z0 = shares()
     ks, tt = dev.time_steps(steps)
+    z1 = shares()
     st = dev.get_state()
-    return ks, tt, st
+    return ks, tt, st, {"start_iteration": reached, "zero_share_current": 0.5 * (z0[0] + z1[0]),
+                        "zero_share_average": 0.5 * (z0[1] + z1[1])}
 
 
 def cpu_baseline(prob, iters: int):
@@ -143,17 +173,39 @@ def roofline(achieved_gbs: float, bytes_per_launch: int, avg_launch_us: float, k
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_us}
 
 
-def step_measurement(dev, n: int, m: int, elem: int, steps: int, warmup: int, matrix_streams: int = 4) -> dict:
-    """K timed take_step! calls from the zero start: iteration rate and the
-    step kernel's HIP-event average against its algorithmic bytes."""
-    ks, tt, st = measure_steps(dev, steps, warmup)
-    alg = algorithmic_bytes_per_step(n, m, elem, matrix_streams)
+def sparse_bytes_per_step(dev, n: int, m: int, elem: int, stored_c: bool, z_cur: float, z_avg: float) -> int:
+    """Bytes one PDHG iteration moves on average with zero maps: C in full (the
+    stored form); X read and X+ written only in the 16-byte units not recorded
+    as +0.0 (share z_cur); the average, read and written by every second pass,
+    only in its nonzero units (share z_avg); the maps themselves (one 64-bit
+    word per wave and column: the current point's read and the new point's
+    written every pass, the average's two every second pass); the row and
+    column vectors.  With no zeros this is the dense figure plus the maps."""
+    L = dev.get_launch()
+    vec = 16 // elem
+    waves = L["rows_per_block"] // (64 * vec)
+    row_blocks = L["blocks"] // L["col_blocks"]
+    map_bytes = 8 * row_blocks * waves * L["col_blocks"] * L["cols_per_block"]
+    mats = n * m * elem * ((1 if stored_c else 0) + 2 * (1 - z_cur) + (1 - z_avg))
+    return int(mats + 3 * map_bytes + 10 * 8 * (n + m))
+
+
+def step_measurement(dev, n: int, m: int, elem: int, steps: int, warmup: int, matrix_streams: int = 4,
+                     prob=None, storage=None, start: int = 0) -> dict:
+    """K timed take_step! calls (after `start` solve iterations of `prob` and
+    `warmup` untimed passes): iteration rate and the step kernel's HIP-event
+    average against the bytes it had to move (zero maps: the units recorded
+    as +0.0 are neither read nor written), and the dense figure beside it."""
+    ks, tt, st, z = measure_steps(dev, steps, warmup, prob, storage, start)
+    alg = sparse_bytes_per_step(dev, n, m, elem, matrix_streams == 4, z["zero_share_current"], z["zero_share_average"])
+    dense = algorithmic_bytes_per_step(n, m, elem, matrix_streams)
     k_avg = ks / steps * 1e-3
-    return {"ms_per_step": tt / steps, "iters_per_s": steps / (tt * 1e-3), "step_kernel_us": k_avg * 1e6,
-            "step_kernel_GBs": alg / k_avg / 1e9, "bytes_per_step": alg, "numerical_error": int(st.numerical_error)}
+    return dict(z, ms_per_step=tt / steps, iters_per_s=steps / (tt * 1e-3), step_kernel_us=k_avg * 1e6,
+                step_kernel_GBs=alg / k_avg / 1e9, bytes_per_step=alg, dense_bytes_per_step=dense,
+                dense_equivalent_GBs=dense / k_avg / 1e9, numerical_error=int(st.numerical_error))
 
 
-def l1_mixture(n: int, steps: int, warmup: int, solve: bool = True, point_cloud: bool = True) -> dict:
+def l1_mixture(n: int, steps: int, warmup: int, solve: bool = True, point_cloud: bool = True, start: int = 0) -> dict:
     """n x n L1-cost mixture (c3 at n = 30000): dense FP32 storage (C, X and
     X_avg 3.6 GB each at 30000) with FP64 duals, and the FP64 form on the same
     instance for the objective error of the FP32-storage result."""
@@ -171,7 +223,7 @@ def l1_mixture(n: int, steps: int, warmup: int, solve: bool = True, point_cloud:
         t0 = time.time()
         dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
         res = {"upload_sec": time.time() - t0, "device_GB": dev.device_bytes() / 1e9}
-        res.update(step_measurement(dev, n, m, elem, steps, warmup))
+        res.update(step_measurement(dev, n, m, elem, steps, warmup, prob=prob, storage=storage, start=start))
         if solve:
             res["solve"] = solve_to_tol(prob, storage, 1e-4, dev=dev, do_round=False)
         out[name] = res
@@ -184,7 +236,8 @@ def l1_mixture(n: int, steps: int, warmup: int, solve: bool = True, point_cloud:
         t0 = time.time()
         dev.upload_points(pts)
         res = {"upload_sec": time.time() - t0, "device_GB": dev.device_bytes() / 1e9}
-        res.update(step_measurement(dev, n, m, 4, steps, warmup, matrix_streams=3))
+        res.update(step_measurement(dev, n, m, 4, steps, warmup, matrix_streams=3, prob=pts,
+                                    storage=_lib.STORAGE_F32, start=start))
         if solve:
             res["solve"] = solve_to_tol(pts, _lib.STORAGE_F32, 1e-4, dev=dev, do_round=False)
         out["f32_point_cloud"] = res
@@ -202,7 +255,7 @@ def l1_mixture(n: int, steps: int, warmup: int, solve: bool = True, point_cloud:
 
 def bench_l1_mixture(args) -> int:
     n = 30000 if args.n == 10000 else args.n
-    print(json.dumps(l1_mixture(n, args.steps, args.warmup, solve=not args.no_solve)))
+    print(json.dumps(l1_mixture(n, args.steps, args.warmup, solve=not args.no_solve, start=args.start_iteration)))
     return 0
 
 
@@ -277,7 +330,7 @@ def bench_batched(args) -> int:
     return 0
 
 
-def cpu_c1(tol: float = 1e-4) -> dict:
+def cpu_c1(tol: float = 1e-4, start: int = 0) -> dict:
     """Config c1: 1000 x 1000 uniform points, squared Euclidean, FP64, solved
     to 1e-4 by the reference algorithm restated on host arrays (the oracle's
     literal mode, OpenMP over the host cores), and by the GPU on the same
@@ -304,7 +357,7 @@ def cpu_c1(tol: float = 1e-4) -> dict:
                    "evaluation, as the reference's cumulative_time_sec"}
     dev = _lib.DeviceSolver(1000, 1000, storage=_lib.STORAGE_F64)
     dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
-    g = step_measurement(dev, 1000, 1000, 8, 200, 20)
+    g = step_measurement(dev, 1000, 1000, 8, 200, 20, prob=prob, storage=_lib.STORAGE_F64, start=start)
     g["solve"] = solve_to_tol(prob, _lib.STORAGE_F64, tol, dev=dev, do_round=False)
     dev.close()
     out["gpu_f64"] = g
@@ -324,7 +377,7 @@ def tight_solve(prob, storage, tol: float) -> dict:
     return res
 
 
-def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_factor: int = 8) -> dict:
+def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_factor: int = 8, start: int = 0) -> dict:
     """Held-out 4096 x 4096 Gaussian-mixture instances (FP32 storage), time
     and iterations to 1e-4 from each starting state:
       default_weight      zero start, the reference's initial primal weight 1.0;
@@ -420,7 +473,7 @@ def warm_start_table(count: int, steps: int = 100, warmup: int = 10, limit_facto
     inst = instances.gmm_dirichlet(4096, seed=900200)
     dev = _lib.DeviceSolver(4096, 4096, storage=_lib.STORAGE_F32)
     dev.upload(inst.problem.cost_matrix, inst.problem.source_distribution, inst.problem.target_distribution)
-    g = step_measurement(dev, 4096, 4096, 4, steps, warmup)
+    g = step_measurement(dev, 4096, 4096, 4, steps, warmup, prob=inst.problem, storage=_lib.STORAGE_F32, start=start)
     dev.close()
     return {"metric": "learned warm start: time-to-1e-4 on held-out 4096x4096 GMM instances (FP32 storage)",
             "unit": "s", "higher_is_better": False, "data": "synthetic", "n_gpus": 1,
@@ -447,7 +500,7 @@ def solve_to_tol_params(prob, storage, params, dev, warm_start=None, duals_out=N
 
 
 def bench_warm_start(args) -> int:
-    print(json.dumps(warm_start_table(args.instances)))
+    print(json.dumps(warm_start_table(args.instances, start=args.start_iteration)))
     return 0
 
 
@@ -557,7 +610,7 @@ def extras(args, prob, f32_roofline, f64_objective=None) -> dict:
         print(f"# {name} done in {time.time() - t0:.1f}s", file=sys.stderr, flush=True)
 
     if not args.no_cpu:
-        step("c1_cpu_1000", cpu_c1)
+        step("c1_cpu_1000", lambda: cpu_c1(start=args.start_iteration))
 
     def c2():
         r = tight_solve(prob, _lib.STORAGE_F32, 1e-6)
@@ -576,9 +629,10 @@ def extras(args, prob, f32_roofline, f64_objective=None) -> dict:
         return r
     if f64_objective is not None:
         step("c2_precision", precision)
-    step("c3_l1_mixture_30k", lambda: l1_mixture(30000, max(10, args.steps // 2), args.warmup, point_cloud=False))
+    step("c3_l1_mixture_30k", lambda: l1_mixture(30000, max(10, args.steps // 2), args.warmup, point_cloud=False,
+                                                  start=args.start_iteration))
     step("c4_batched_512", lambda: batched(args.steps))
-    step("c5_warm_start_4096", lambda: warm_start_table(args.instances))
+    step("c5_warm_start_4096", lambda: warm_start_table(args.instances, start=args.start_iteration))
     return out
 
 
@@ -587,6 +641,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--start-iteration", type=int, default=2048,
+                    help="solve iterations (tolerance 1e-4, untimed) before the warmup and timed passes, so that "
+                         "these see the iterates' zeros as they are at that point of a solve (the zero start "
+                         "has no nonzero entries)")
     ap.add_argument("--n", type=int, default=10000)
     ap.add_argument("--no-solve", action="store_true", help="skip the time-to-tolerance solves (and the extras)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baselines")
@@ -628,7 +686,8 @@ def main() -> int:
         t0 = time.time()
         dev.upload(prob.cost_matrix, prob.source_distribution, prob.target_distribution)
         up = time.time() - t0
-        results[name] = step_measurement(dev, n, m, elem, args.steps, args.warmup)
+        results[name] = step_measurement(dev, n, m, elem, args.steps, args.warmup, prob=prob, storage=storage,
+                                         start=args.start_iteration)
         results[name].update({"iteration_GBs": results[name]["bytes_per_step"] / (results[name]["ms_per_step"] * 1e-3) / 1e9,
                               "launch": dev.get_launch(), "upload_sec": up, "upload_GBs_pcie": 8 * n * m / up / 1e9})
         devs[name] = dev
@@ -639,7 +698,8 @@ def main() -> int:
     for storage, name, elem in ((_lib.STORAGE_F64, "f64", 8), (_lib.STORAGE_F32, "f32", 4)):
         dev = _lib.DeviceSolver(n, m, storage=storage)
         dev.upload_points(pts)
-        points[name] = step_measurement(dev, n, m, elem, args.steps, args.warmup, matrix_streams=3)
+        points[name] = step_measurement(dev, n, m, elem, args.steps, args.warmup, matrix_streams=3, prob=pts,
+                                        storage=storage, start=args.start_iteration)
         points[name]["device_GB"] = dev.device_bytes() / 1e9
         if not args.no_solve:
             points[name]["time_to_1e-4"] = solve_to_tol(pts, storage, 1e-4, dev=dev)

@@ -264,12 +264,13 @@ int pdot_set_stream_policy(pdot_ctx* ctx, int32_t mode);
  * choice of this library). */
 int pdot_set_sparse(pdot_ctx* ctx, int32_t enable);
 
-/* Share of the point's 16-byte column units (`which`: PDOT_POINT_CURRENT or
- * PDOT_POINT_AVERAGE, whose lagging contribution is written out first) that
- * its zero map records as +0.0, i.e. that passes neither read nor write; 0
- * when zero maps are off.  A unit is recorded as +0.0 only when it is; the
- * share is exact for a point written by a pass with zero maps on.  No
- * counterpart in the reference (a diagnostic of this library). */
+/* Share of the 16-byte column units of the slot holding the point (`which`:
+ * PDOT_POINT_CURRENT or PDOT_POINT_AVERAGE; the average's slot as the next
+ * pass reads it, without a lagging contribution) that its zero map records
+ * as +0.0, i.e. that passes do not read; 0 when zero maps are off.  A unit
+ * is recorded as +0.0 only when it is; the share is exact for a slot written
+ * by a pass with zero maps on.  Changes no state.  No counterpart in the
+ * reference (a diagnostic of this library). */
 int pdot_zero_share(pdot_ctx* ctx, int32_t which, double* share);
 
 /* Launch geometry of the streaming step kernel: rows per block, column

@@ -168,7 +168,6 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             a.zc = a.za = nullptr;
             a.zn = a.zan = nullptr;
             a.zstride = 0;
-            a.zpage = nullptr;
             step_tile<T, EVAL, kBUnroll, kBWaves, kBPipe, kCostDense, kKindAny, true, false>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
                 const int64_t lb = stl.rb + (int64_t)stl.cb * sinst.R;

@@ -495,10 +495,6 @@ __device__ __forceinline__ void store_vec(float* p, const double* o) {
     st16<NT>(v, reinterpret_cast<flt4*>(p));
 }
 
-// Lanes per group whose 16-byte stores a sparse pass makes or skips
-// together (kZGroup * 16 = 64 bytes of a column; quad_any below).
-constexpr int kZGroup = 4;
-
 // True unless every value is +0.0 (a -0.0 counts as a value).
 template <int N>
 __device__ __forceinline__ bool any_bits(const double* o) {
@@ -514,12 +510,10 @@ __device__ __forceinline__ bool lane_bit(uint64_t word, int lane) {
     return (half >> (lane & 31)) & 1;
 }
 
-// Whether any lane of this lane's group of four has x set (all lanes active).
-__device__ __forceinline__ bool quad_any(bool x) {
-    int v = x ? 1 : 0;
-    v |= __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
-    v |= __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
-    return v != 0;
+// Set bits of a wave-uniform word below this lane: where this lane's 16
+// bytes sit in a compacted segment (see TileArgs).
+__device__ __forceinline__ int slot_in_segment(uint64_t word) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(word >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)word, 0u));
 }
 
 // ------------------------------------------------------------ step tile
@@ -615,10 +609,12 @@ struct TileArgs {
     bool defer, pend;
     double pnw, paw;
     // Zero maps (sparse passes): one 64-bit word per wave and tile column,
-    // bit L for lane L's VEC rows.  A clear bit says those rows of that
-    // column are +0.0 in that slot, whatever its memory holds: they are not
-    // read, and a new point's rows are stored only where a group of
-    // kZGroup lanes holds a value other than +0.0.  Wave w's word for tile
+    // bit L for lane L's VEC rows (16 bytes).  A clear bit says those rows
+    // of that column are +0.0 in that slot; the set bits' 16-byte units are
+    // stored compacted at the front of the wave's segment of the column (its
+    // 64 * VEC rows from the wave's first row), lane L's at position
+    // popcount(bits below L), and nothing else of the segment is read or
+    // written.  All bits set is the dense layout.  Wave w's word for tile
     // column t is at z* + w * zstride + t.
     bool sparse;
     const uint64_t* zc;  // current point's map
@@ -626,7 +622,6 @@ struct TileArgs {
     uint64_t* zn;        // new point's map
     uint64_t* zan;       // new average's map
     int64_t zstride;
-    const T* zpage;      // 16 bytes of +0.0
 };
 
 // One PDHG pass over a tile (src/solver_gpu.jl:125-174 and 237-264,
@@ -746,13 +741,18 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             const uint64_t wc = zwc[q], wa = zwa[q];
             if (colok && lane_active) {
                 const int64_t idx = j * ld + i0;
-                // a lane whose rows are +0.0 loads the zero page instead
-                const T* xp = lane_bit(wc, lane) ? Xc + idx : a.zpage;
-                B.xv[q] = ld16<NT>(reinterpret_cast<const V*>(xp));
+                const int64_t seg = idx - (int64_t)lane * VEC;  // the wave's segment of column j
+                // a lane whose rows are recorded as +0.0 issues no load
+                V xv = {};
+                if (lane_bit(wc, lane))
+                    xv = ld16<NT>(reinterpret_cast<const V*>(Xc + seg + (int64_t)slot_in_segment(wc) * VEC));
+                B.xv[q] = xv;
                 if constexpr (COST == kCostDense) B.cv[q] = ld16<NT>(reinterpret_cast<const V*>(C + idx));
                 if (!skip_avg) {
-                    const T* ap = lane_bit(wa, lane) ? Xa + idx : a.zpage;
-                    B.av[q] = ld16<NT>(reinterpret_cast<const V*>(ap));
+                    V av = {};
+                    if (lane_bit(wa, lane))
+                        av = ld16<NT>(reinterpret_cast<const V*>(Xa + seg + (int64_t)slot_in_segment(wa) * VEC));
+                    B.av[q] = av;
                 }
             }
         }
@@ -841,13 +841,12 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             // block-uniform: all lanes take part in the ballots
             const uint64_t bn = sparse ? __builtin_amdgcn_ballot_w64(live) : ~0ull;
             const uint64_t ban = sparse ? __builtin_amdgcn_ballot_w64(alive) : ~0ull;
-            const bool gn = !sparse || quad_any(live);
-            const bool gan = !sparse || quad_any(alive);
             if (do_step && colok) {
-                if (lane_active) {
-                    const int64_t idx = j * ld + i0;
-                    if (gn) store_vec<NT>(Xn + idx, xn);
-                    if (!skip_avg && gan) store_vec<NT>(Xan + idx, xan);
+                if (lane_active) {  // the new points compacted by their own maps
+                    const int64_t seg = j * ld + i0 - (int64_t)lane * VEC;
+                    if (lane_bit(bn, lane)) store_vec<NT>(Xn + seg + (int64_t)slot_in_segment(bn) * VEC, xn);
+                    if (!skip_avg && lane_bit(ban, lane))
+                        store_vec<NT>(Xan + seg + (int64_t)slot_in_segment(ban) * VEC, xan);
                 }
                 if (sparse && lane == 0) {
                     zn[tb + q] = bn;

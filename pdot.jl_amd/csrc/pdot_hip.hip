@@ -83,7 +83,6 @@ struct StepParams {
     uint64_t* zmap[4];   // zero map of each X slot (see ZGeom)
     int64_t ztc;         // map words per wave and tile: the most columns a tile has
     int32_t sparse;      // passes read and write through the zero maps
-    const void* zpage;   // 16 bytes of +0.0
 };
 
 struct ReduceParams {
@@ -185,7 +184,6 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     a.zn = p.zmap[sst.spare0] + zbase;
     a.zan = p.zmap[sst.spare1] + zbase;
     a.zstride = p.ztc;
-    a.zpage = static_cast<const T*>(p.zpage);
     double dd = 0.0, ev[kMatStats];
     step_tile<T, EVAL, UNROLL, WAVES, PIPE, COST, KIND, NT>(a, colbuf, red, &dd, ev);
     if (tid == 0) {
@@ -464,10 +462,15 @@ struct ZGeom {
     int64_t words = 0;  // R * Cb * waves * tc
 };
 
-// Writes +0.0 into every lane whose map bit is clear, so that the slot's
-// memory holds every value (the map stays valid).  One wave per word.
+// Expands every wave segment of a slot to the dense layout in place: lane L
+// takes its 16 bytes from position popcount(bits below L) of the segment, or
+// +0.0 where its bit is clear, so that the slot's memory holds every value in
+// column-major order (the map must then be set to all ones, which describes
+// that layout).  One wave per word; every lane's load completes before any
+// lane of the wave stores.
 template <typename T>
 __global__ __launch_bounds__(256) void densify_kernel(T* __restrict__ X, const uint64_t* __restrict__ z, ZGeom g) {
+    using V = typename VecT<T>::type;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
     for (int64_t wi = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < g.words; wi += nw) {
@@ -487,9 +490,14 @@ __global__ __launch_bounds__(256) void densify_kernel(T* __restrict__ X, const u
         }
         const int64_t row0 = rb * g.TRB;
         const int64_t row_end = (row0 + g.TRB < g.n) ? row0 + g.TRB : g.n;
-        const int64_t i0 = row0 + (w * kWave + lane) * g.vec;
-        if (t >= ncols || i0 >= row_end || ((z[wi] >> lane) & 1)) continue;
-        for (int e = 0; e < g.vec; ++e) X[j * g.ld + i0 + e] = (T)0;
+        const int64_t seg = j * g.ld + row0 + w * kWave * g.vec;  // wave-uniform
+        const uint64_t word = z[wi];
+        if (t >= ncols || word == ~0ull) continue;  // wave-uniform: past the tile, or dense already
+        const bool active = row0 + (w * kWave + lane) * g.vec < row_end;
+        V v = {};
+        if (active && lane_bit(word, lane)) v = *reinterpret_cast<const V*>(X + seg + (int64_t)slot_in_segment(word) * g.vec);
+        wait_memory_ops();
+        if (active) *reinterpret_cast<V*>(X + seg + (int64_t)lane * g.vec) = v;
     }
 }
 
@@ -573,7 +581,6 @@ struct pdot_ctx {
     bool sparse = true;           // passes skip the zero rows recorded in the zero maps
     uint64_t* zmap[4] = {nullptr, nullptr, nullptr, nullptr};  // per X slot (geometry zg)
     ZGeom zg;
-    void* zpage = nullptr;        // 256 bytes of +0.0: what the lanes of known zeros load
     int32_t* kind_err = nullptr;  // device flag, see step_kernel
     uint32_t* done = nullptr;     // reduce_kernel's completion count, see there
     std::map<int64_t, hipGraphExec_t> graphs;
@@ -634,7 +641,10 @@ void free_partials(pdot_ctx* ctx) {
     }
 }
 
-// Zero maps.  Makes slot `slot`'s memory hold every value (its map stays valid).
+int fill_map(pdot_ctx* ctx, int slot, uint64_t value);
+
+// Zero maps.  Makes slot `slot`'s memory hold every value in the dense
+// layout, and its map all ones (which describes that layout).
 int densify_slot(pdot_ctx* ctx, int slot) {
     if (!ctx->zmap[slot] || ctx->zg.words == 0) return PDOT_OK;
     const ZGeom& g = ctx->zg;
@@ -646,7 +656,7 @@ int densify_slot(pdot_ctx* ctx, int slot) {
         hipLaunchKernelGGL(densify_kernel<float>, dim3(blocks), dim3(256), 0, ctx->stream,
                            static_cast<float*>(ctx->s.X[slot]), ctx->zmap[slot], g);
     HIPCHK(ctx, hipGetLastError());
-    return PDOT_OK;
+    return fill_map(ctx, slot, ~0ull);
 }
 
 int fill_map(pdot_ctx* ctx, int slot, uint64_t value) {
@@ -785,7 +795,6 @@ StepParams step_params(pdot_ctx* ctx) {
     for (int k = 0; k < 4; ++k) p.zmap[k] = ctx->zmap[k];
     p.ztc = ctx->zg.tc;
     p.sparse = (ctx->sparse && ctx->zg.words > 0) ? 1 : 0;
-    p.zpage = ctx->zpage;
     return p;
 }
 
@@ -979,7 +988,6 @@ int materialize_average(pdot_ctx* ctx, pdot_solver_state& st) {
     if (ctx->sparse) {  // the kernel below reads and writes every value
         if ((rc = densify_slot(ctx, st.avg_slot))) return rc;
         if ((rc = densify_slot(ctx, st.cur_slot))) return rc;
-        if ((rc = fill_map(ctx, st.avg_slot, ~0ull))) return rc;
     }
     if (ctx->storage == PDOT_STORAGE_F64)
         hipLaunchKernelGGL(materialize_kernel<double>, dim3(4096), dim3(256), 0, ctx->stream,
@@ -1129,8 +1137,6 @@ int pdot_create(pdot_ctx** out, int64_t n, int64_t m, int32_t storage, int32_t d
     if ((rc = dev_alloc(ctx, (void**)&ctx->b, sizeof(double) * (size_t)m))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->evalout, sizeof(double) * kEvalOut))) return rc;
     if ((rc = dev_alloc(ctx, (void**)&ctx->stA, sizeof(pdot_solver_state)))) return rc;
-    if ((rc = dev_alloc(ctx, &ctx->zpage, 256))) return rc;
-    HIPCHK(ctx, hipMemsetAsync(ctx->zpage, 0, 256, ctx->stream));
     if ((rc = dev_alloc(ctx, (void**)&ctx->kind_err, sizeof(int32_t)))) return rc;
     HIPCHK(ctx, hipMemsetAsync(ctx->kind_err, 0, sizeof(int32_t), ctx->stream));
     if ((rc = dev_alloc(ctx, (void**)&ctx->done, sizeof(uint32_t)))) return rc;
@@ -1149,7 +1155,7 @@ void pdot_destroy(pdot_ctx* ctx) {
     for (hipEvent_t e : ctx->events) hipEventDestroy(e);
     free_partials(ctx);
     free_maps(ctx);
-    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->done, ctx->stage, ctx->zpage};
+    void* ps[] = {ctx->C, ctx->a, ctx->b, ctx->evalout, ctx->stA, ctx->kind_err, ctx->done, ctx->stage};
     for (void* q : ps)
         if (q) hipFree(q);
     for (int s = 0; s < 4; ++s) {
@@ -1681,8 +1687,7 @@ int pdot_zero_share(pdot_ctx* ctx, int32_t which, double* share) {
     if (which == PDOT_POINT_CURRENT) slot = st.cur_slot;
     else if (which == PDOT_POINT_AVERAGE) {
         if (st.avg_count == 0) return fail(ctx, PDOT_E_STATE, "pdot_zero_share: the average is empty");
-        if ((rc = materialize_average(ctx, st))) return rc;
-        slot = st.avg_slot;
+        slot = st.avg_slot;  // as the next averaging pass reads it (a lagging contribution stays lagging)
     } else return fail(ctx, PDOT_E_ARG, "pdot_zero_share: unknown point");
     *share = 0.0;
     if (!ctx->sparse || ctx->zg.words == 0) return PDOT_OK;
