@@ -32,7 +32,7 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 // (stacked along rows), columns per load batch, and how many batches are
 // prefetched ahead of the one computed.
 #ifndef PDOT_F64_WAVES
-#define PDOT_F64_WAVES 8
+#define PDOT_F64_WAVES 4
 #endif
 #ifndef PDOT_F64_UNROLL
 #define PDOT_F64_UNROLL 2
@@ -50,7 +50,7 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 #define PDOT_F32_PIPE 0
 #endif
 #ifndef PDOT_F64P_WAVES
-#define PDOT_F64P_WAVES PDOT_F64_WAVES
+#define PDOT_F64P_WAVES PDOT_F64_WAVES  // the dense geometry: FP64 point solves equal dense ones bit for bit
 #endif
 #ifndef PDOT_F64P_UNROLL
 #define PDOT_F64P_UNROLL 2  // two columns per batch, no prefetch: measured best with deferred averaging
