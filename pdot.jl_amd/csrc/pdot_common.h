@@ -646,7 +646,10 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
     const int lane = tid & (kWave - 1);
     const int w = tid / kWave;
     const int64_t n = a.n, ld = a.ld;
-    const int64_t i0 = a.row0 + (int64_t)w * WROWS + (int64_t)lane * VEC;
+    // the wave's first row (wave-uniform) and this lane's offset from it
+    const int64_t wrow0 = a.row0 + (int64_t)__builtin_amdgcn_readfirstlane(w) * WROWS;
+    const uint32_t loff = (uint32_t)lane * VEC;
+    const int64_t i0 = wrow0 + loff;
     const int64_t row_end = a.row_end;
     const bool lane_active = i0 < row_end;
     const bool avg_empty = a.avg_empty;
@@ -740,18 +743,17 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             }
             const uint64_t wc = zwc[q], wa = zwa[q];
             if (colok && lane_active) {
-                const int64_t idx = j * ld + i0;
-                const int64_t seg = idx - (int64_t)lane * VEC;  // the wave's segment of column j
+                const int64_t seg = j * ld + wrow0;  // the wave's segment of column j (wave-uniform)
                 // a lane whose rows are recorded as +0.0 issues no load
                 V xv = {};
-                if (lane_bit(wc, lane))
-                    xv = ld16<NT>(reinterpret_cast<const V*>(Xc + seg + (int64_t)slot_in_segment(wc) * VEC));
+                if (__builtin_amdgcn_inverse_ballot_w64(wc))
+                    xv = ld16<NT>(reinterpret_cast<const V*>(Xc + seg) + (uint32_t)slot_in_segment(wc));
                 B.xv[q] = xv;
-                if constexpr (COST == kCostDense) B.cv[q] = ld16<NT>(reinterpret_cast<const V*>(C + idx));
+                if constexpr (COST == kCostDense) B.cv[q] = ld16<NT>(reinterpret_cast<const V*>(C + seg + loff));
                 if (!skip_avg) {
                     V av = {};
-                    if (lane_bit(wa, lane))
-                        av = ld16<NT>(reinterpret_cast<const V*>(Xa + seg + (int64_t)slot_in_segment(wa) * VEC));
+                    if (__builtin_amdgcn_inverse_ballot_w64(wa))
+                        av = ld16<NT>(reinterpret_cast<const V*>(Xa + seg) + (uint32_t)slot_in_segment(wa));
                     B.av[q] = av;
                 }
             }
@@ -843,10 +845,11 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
             const uint64_t ban = sparse ? __builtin_amdgcn_ballot_w64(alive) : ~0ull;
             if (do_step && colok) {
                 if (lane_active) {  // the new points compacted by their own maps
-                    const int64_t seg = j * ld + i0 - (int64_t)lane * VEC;
-                    if (lane_bit(bn, lane)) store_vec<NT>(Xn + seg + (int64_t)slot_in_segment(bn) * VEC, xn);
-                    if (!skip_avg && lane_bit(ban, lane))
-                        store_vec<NT>(Xan + seg + (int64_t)slot_in_segment(ban) * VEC, xan);
+                    const int64_t seg = j * ld + wrow0;
+                    if (__builtin_amdgcn_inverse_ballot_w64(bn))
+                        store_vec<NT>(Xn + seg + (uint32_t)slot_in_segment(bn) * VEC, xn);
+                    if (!skip_avg && __builtin_amdgcn_inverse_ballot_w64(ban))
+                        store_vec<NT>(Xan + seg + (uint32_t)slot_in_segment(ban) * VEC, xan);
                 }
                 if (sparse && lane == 0) {
                     zn[tb + q] = bn;
