@@ -173,7 +173,7 @@ def roofline(achieved_gbs: float, bytes_per_launch: int, avg_launch_us: float, k
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_us}
 
 
-def sparse_bytes_per_step(dev, n: int, m: int, elem: int, stored_c: bool, z_cur: float, z_avg: float) -> int:
+def sparse_bytes_per_step(launch: dict, n: int, m: int, elem: int, stored_c: bool, z_cur: float, z_avg: float) -> int:
     """Bytes one PDHG iteration moves on average with zero maps: C in full (the
     stored form); X read and X+ written only in the 16-byte units not recorded
     as +0.0 (share z_cur); the average, read and written by every second pass,
@@ -181,7 +181,7 @@ def sparse_bytes_per_step(dev, n: int, m: int, elem: int, stored_c: bool, z_cur:
     word per wave and column: the current point's read and the new point's
     written every pass, the average's two every second pass); the row and
     column vectors.  With no zeros this is the dense figure plus the maps."""
-    L = dev.get_launch()
+    L = launch
     vec = 16 // elem
     waves = L["rows_per_block"] // (64 * vec)
     row_blocks = L["blocks"] // L["col_blocks"]
@@ -197,10 +197,11 @@ def step_measurement(dev, n: int, m: int, elem: int, steps: int, warmup: int, ma
     average against the bytes it had to move (zero maps: the units recorded
     as +0.0 are neither read nor written), and the dense figure beside it."""
     ks, tt, st, z = measure_steps(dev, steps, warmup, prob, storage, start)
-    alg = sparse_bytes_per_step(dev, n, m, elem, matrix_streams == 4, z["zero_share_current"], z["zero_share_average"])
+    launch = dev.get_launch()
+    alg = sparse_bytes_per_step(launch, n, m, elem, matrix_streams == 4, z["zero_share_current"], z["zero_share_average"])
     dense = algorithmic_bytes_per_step(n, m, elem, matrix_streams)
     k_avg = ks / steps * 1e-3
-    return dict(z, ms_per_step=tt / steps, iters_per_s=steps / (tt * 1e-3), step_kernel_us=k_avg * 1e6,
+    return dict(z, launch=launch, ms_per_step=tt / steps, iters_per_s=steps / (tt * 1e-3), step_kernel_us=k_avg * 1e6,
                 step_kernel_GBs=alg / k_avg / 1e9, bytes_per_step=alg, dense_bytes_per_step=dense,
                 dense_equivalent_GBs=dense / k_avg / 1e9, numerical_error=int(st.numerical_error))
 
@@ -758,6 +759,9 @@ def main() -> int:
                              "step_kernel<double>", measured_traffic(n, m, "f64")),
         "cpu_baseline": cpu,
         "iteration_GBs": f64["iteration_GBs"],
+        "f64_step": {k: f64[k] for k in ("start_iteration", "zero_share_current", "zero_share_average",
+                                          "bytes_per_step", "dense_bytes_per_step", "dense_equivalent_GBs",
+                                          "step_kernel_us", "launch")},
         "f32_storage": dict(f32, roofline=f32_roof),
         "time_to_1e-4": solves or None,
         "point_cloud_cost": points,

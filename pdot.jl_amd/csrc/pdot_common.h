@@ -35,10 +35,10 @@ constexpr int kEvalOut = kMatStats + kVecStats;
 #define PDOT_F64_WAVES 4
 #endif
 #ifndef PDOT_F64_UNROLL
-#define PDOT_F64_UNROLL 2
+#define PDOT_F64_UNROLL 1
 #endif
 #ifndef PDOT_F64_PIPE
-#define PDOT_F64_PIPE 0
+#define PDOT_F64_PIPE 1
 #endif
 #ifndef PDOT_F32_WAVES
 #define PDOT_F32_WAVES 8
@@ -119,6 +119,19 @@ template <>
 struct LeaveShape<float, true> {
     static constexpr bool kOwnKernel = true;
     static constexpr int kUnroll = PDOT_F32P_LEAVE_UNROLL, kPipe = PDOT_F32P_LEAVE_PIPE;
+};
+#ifndef PDOT_DENSE_LEAVE
+#define PDOT_DENSE_LEAVE 1
+#endif
+template <>
+struct LeaveShape<double, false> {
+    static constexpr bool kOwnKernel = PDOT_DENSE_LEAVE;
+    static constexpr int kUnroll = PDOT_F64_UNROLL, kPipe = PDOT_F64_PIPE;
+};
+template <>
+struct LeaveShape<float, false> {
+    static constexpr bool kOwnKernel = PDOT_DENSE_LEAVE;
+    static constexpr int kUnroll = PDOT_F32_UNROLL, kPipe = PDOT_F32_PIPE;
 };
 
 // ------------------------------------------------------------------ helpers
