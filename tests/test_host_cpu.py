@@ -111,7 +111,7 @@ def test_bench_byte_accounting_matches_the_traffic_profiles():
     element streams on average with deferred averaging, 3 for point clouds)
     and the bytes the passes must move with zero maps at the recorded zero
     shares.  The traffic lies between the two: at least the needed bytes
-    (partial sums and cache-line rounding on top, under 25%), far below dense."""
+    (partial sums and cache-line rounding on top, under 30%), far below dense."""
     import importlib.util
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(REPO_ROOT, "bench.py"))
     bench = importlib.util.module_from_spec(spec)
@@ -131,7 +131,7 @@ def test_bench_byte_accounting_matches_the_traffic_profiles():
         # no zeros: the dense figure plus the maps
         nozero = bench.sparse_bytes_per_step(r["launch"], n, n, elem, streams == 4, 0.0, 0.0)
         assert dense < nozero < 1.01 * dense, form
-        assert 1.0 <= traffic / needed < 1.25, form
+        assert 1.0 <= traffic / needed < 1.3, form
         assert traffic < 0.5 * dense, form
     assert bench.measured_traffic(5000, 5000, "f64") is None
     bt = bench.measured_batch_traffic()
