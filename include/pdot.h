@@ -232,9 +232,9 @@ int pdot_set_defer(pdot_ctx* ctx, int32_t enable);
 
 /* Passes that leave the average alone (every second ordinary pass with
  * deferred averaging) run a kernel compiled for that kind, without the
- * average's loads and registers, where the form has one (point clouds with
- * FP32 storage); enable = 0 runs the kernel that decides at run time for
- * every pass, for comparison.  Results are identical either way.  No
+ * average's loads and registers, where the form has one (stored costs in FP64
+ * and FP32, point clouds with FP32 storage); enable = 0 runs the kernel that
+ * decides at run time for every pass, for comparison.  Results are identical either way.  No
  * counterpart in the reference (a kernel-selection choice of this library). */
 int pdot_set_pass_kernels(pdot_ctx* ctx, int32_t enable);
 
