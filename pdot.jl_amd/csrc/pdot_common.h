@@ -163,8 +163,15 @@ template <int CTRL, int ROW_MASK>
 __device__ __forceinline__ double dpp_f64(double x) {
     int lo = __double2loint(x);
     int hi = __double2hiint(x);
-    lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xF, false);
-    hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xF, false);
+    if constexpr (ROW_MASK == 0xF) {
+        // every lane is written, from a lane of its own row: no old value to set
+        lo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xF, 0xF, true);
+        hi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xF, 0xF, true);
+    } else {
+        // rows left out keep 0
+        lo = __builtin_amdgcn_update_dpp(0, lo, CTRL, ROW_MASK, 0xF, false);
+        hi = __builtin_amdgcn_update_dpp(0, hi, CTRL, ROW_MASK, 0xF, false);
+    }
     return __hiloint2double(hi, lo);
 }
 __device__ __forceinline__ double readlane_f64(double x, int lane) {
