@@ -277,6 +277,9 @@ def batched(steps: int, solve: bool = True) -> dict:
     for k, p in enumerate(probs):
         b.upload(k, p.cost_matrix, p.source_distribution, p.target_distribution)
     up = time.time() - t0
+    # the pass timing below starts from zeros, which the zero maps would skip almost
+    # entirely: time the dense passes (every element read and written) for the roofline
+    b.set_sparse(False)
     tune = autotune(b, steps=4)
     b.set_start(_lib.POLICY_ADAPTIVE, 2.0, 1.0, 1.0, 1.0)
     b.time_steps(1)  # the first pass after a start has an empty average (untimed)
@@ -295,6 +298,8 @@ def batched(steps: int, solve: bool = True) -> dict:
            "bytes_per_pass": alg}
     res["roofline"] = roofline(res["step_kernel_GBs"], alg, ks / steps * 1e3,
                                "bstep_kernel<float> (every instance active)", measured_batch_traffic())
+    b.set_sparse(True)  # the solves run with zero maps, the default
+    res["zero_maps"] = True
     if solve:
         from dataclasses import replace
         from pdot_amd import learned

@@ -338,6 +338,12 @@ int pdot_batch_use_graphs(pdot_batch* b, int32_t enable);
  * for a single problem (on by default; bit-identical results).  No
  * counterpart in the reference. */
 int pdot_batch_set_defer(pdot_batch* b, int32_t enable);
+
+/* Zero maps in the batch (on by default), as pdot_set_sparse for every
+ * instance: iterate entries recorded as +0.0 are neither read nor written.
+ * Results are bit for bit those of dense passes; enable = 0 streams every
+ * entry (the slots are expanded first).  No counterpart in the reference. */
+int pdot_batch_set_sparse(pdot_batch* b, int32_t enable);
 /* Launch shape of the persistent step kernel (tuned by pdot_amd.batch.autotune):
  * columns per tile and the grid as a multiple of the resident block count. */
 int pdot_batch_set_launch(pdot_batch* b, int32_t cols_per_tile, int32_t grid_mult);

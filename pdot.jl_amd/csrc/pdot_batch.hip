@@ -79,6 +79,7 @@ struct BInst {
     int64_t v[4], c[4];       // (length m)
     int64_t a, b;
     int64_t rowpart, colpart, sqpart, evalpart;  // offsets into the partial arena
+    int64_t Z[4];             // word offsets of each slot's zero map (R * waves * m words)
     int64_t redpart, evalvec;                    // offsets into the reduction arena
     int32_t R, Cb, TC, nstep, NBr, NBc, nred, sqchunk;
 };
@@ -102,6 +103,8 @@ struct BParams {
     const int32_t* nwork;  // {ntiles, heavy chunks, light chunks, active instances}
     const int32_t* actlist;  // the active instances
     double* evalout;       // kEvalOut per instance
+    uint64_t* zmap;        // the zero maps of every instance and slot
+    int32_t sparse;        // passes read and write through them
 };
 
 template <typename T>
@@ -164,11 +167,18 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
             a.pnw = sst.pend_nw;
             a.paw = sst.pend_aw;
             double dd = 0.0, ev[kMatStats];
-            a.sparse = false;
-            a.zc = a.za = nullptr;
-            a.zn = a.zan = nullptr;
-            a.zstride = 0;
-            step_tile<T, EVAL, kBUnroll, kBWaves, kBPipe, kCostDense, kKindAny, true, false>(a, colbuf, red, &dd, ev);
+            // zero maps: word ((rb * waves + w) * m + j) of a slot's map covers
+            // wave w's rows of column j of row tile rb (fixed tiles of TR rows,
+            // whatever the column tiling)
+            a.sparse = p.sparse != 0;
+            const int cs = sst.cur_slot, as = sst.avg_count == 0 ? cs : sst.avg_slot;
+            const int64_t zrow = (int64_t)stl.rb * kBWaves * sinst.m + a.j0;
+            a.zc = p.zmap + sinst.Z[cs] + zrow;
+            a.za = p.zmap + sinst.Z[as] + zrow;
+            a.zn = p.zmap + sinst.Z[sst.spare0] + zrow;
+            a.zan = p.zmap + sinst.Z[sst.spare1] + zrow;
+            a.zstride = sinst.m;
+            step_tile<T, EVAL, kBUnroll, kBWaves, kBPipe>(a, colbuf, red, &dd, ev);
             if (tid == 0) {
                 const int64_t lb = stl.rb + (int64_t)stl.cb * sinst.R;
                 if (do_step) p.part[sinst.sqpart + lb] = dd;
@@ -410,6 +420,9 @@ struct pdot_batch {
     bool layout_ok = false;  // partial arenas and work lists match the active set
     bool use_graph = false;
     bool defer = true;  // ordinary passes alternate deferring / applying the averages' update
+    bool sparse = true;  // passes skip the zero rows recorded in the zero maps
+    uint64_t* zmap = nullptr;  // every instance's four zero maps (BInst::Z)
+    int64_t zwords = 0;
     std::map<int64_t, hipGraphExec_t> graphs;
     std::vector<hipEvent_t> events;
     int64_t bytes = 0;
@@ -455,6 +468,8 @@ BParams bparams(pdot_batch* b) {
     p.nwork = b->d_nwork;
     p.actlist = b->d_actlist;
     p.evalout = b->d_evalout;
+    p.zmap = b->zmap;
+    p.sparse = (b->sparse && b->zmap) ? 1 : 0;
     return p;
 }
 
@@ -667,11 +682,46 @@ void bchoose_spares(pdot_solver_state& st) {
 // Writes instance k's lagging average contribution (if any) into its average
 // slots, so that they hold the average itself; updates b->h_st[k] only (the
 // caller writes the states back).
+// Zero maps of instance k: the slot's segments expanded to the dense layout in
+// place, and its map set to all ones (which describes that layout).
+int bdensify(pdot_batch* b, int32_t k, int slot) {
+    if (!b->sparse || !b->zmap) return PDOT_OK;
+    const BInst& in = b->inst[k];
+    ZGeom g;
+    g.n = in.n;
+    g.m = in.m;
+    g.ld = in.ld;
+    g.TRB = rows_per_tile(b->storage);
+    g.R = (int32_t)((in.n + g.TRB - 1) / g.TRB);
+    g.Cb = 1;  // one "column block" of all m columns: word (rb * waves + w) * m + j
+    g.TC = (int32_t)in.m;
+    g.interleave = 0;
+    g.waves = kBWaves;
+    g.vec = (int32_t)(16 / b->esz);
+    g.tc = in.m;
+    g.words = (int64_t)g.R * kBWaves * in.m;
+    const int64_t blocks = std::min<int64_t>((g.words + 3) / 4, 4096);
+    uint64_t* z = b->zmap + in.Z[slot];
+    if (b->storage == PDOT_STORAGE_F64)
+        hipLaunchKernelGGL(densify_kernel<double>, dim3(blocks), dim3(256), 0, b->stream,
+                           static_cast<double*>(b->mat) + in.X[slot], z, g);
+    else
+        hipLaunchKernelGGL(densify_kernel<float>, dim3(blocks), dim3(256), 0, b->stream,
+                           static_cast<float*>(b->mat) + in.X[slot], z, g);
+    BHIPCHK(b, hipGetLastError());
+    hipLaunchKernelGGL(fill_u64_kernel, dim3(std::min<int64_t>((g.words + 255) / 256, 1024)), dim3(256), 0, b->stream,
+                       z, g.words, ~0ull);
+    BHIPCHK(b, hipGetLastError());
+    return PDOT_OK;
+}
+
 int bmaterialize(pdot_batch* b, int32_t k) {
     pdot_solver_state& st = b->h_st[k];
     if (!st.avg_pending) return PDOT_OK;
     const BInst& in = b->inst[k];
     const int64_t total = in.ld * in.m;
+    int rc;
+    if ((rc = bdensify(b, k, st.avg_slot)) || (rc = bdensify(b, k, st.cur_slot))) return rc;  // read and written whole
     if (b->storage == PDOT_STORAGE_F64)
         hipLaunchKernelGGL(materialize_kernel<double>, dim3(256), dim3(256), 0, b->stream,
                            static_cast<double*>(b->mat) + in.X[st.avg_slot],
@@ -745,12 +795,20 @@ int pdot_batch_create(pdot_batch** out, int32_t count, const int64_t* n, const i
         }
         in.a = voff; voff += in.ld;
         in.b = voff; voff += in.m;
+        const int64_t rt = (in.n + rows_per_tile(storage) - 1) / rows_per_tile(storage);
+        for (int s = 0; s < 4; ++s) {
+            in.Z[s] = b->zwords;
+            b->zwords += rt * kBWaves * in.m;
+        }
     }
     int rc;
     if ((rc = balloc(b, &b->mat, b->esz * (size_t)moff))) return rc;
     if ((rc = balloc(b, (void**)&b->vec, sizeof(double) * (size_t)voff))) return rc;
     BHIPCHK(b, hipMemsetAsync(b->mat, 0, b->esz * (size_t)moff, b->stream));
     BHIPCHK(b, hipMemsetAsync(b->vec, 0, sizeof(double) * (size_t)voff, b->stream));
+    // all bits set: the dense layout (the slots hold zeros, which every map describes)
+    if ((rc = balloc(b, (void**)&b->zmap, sizeof(uint64_t) * (size_t)b->zwords))) return rc;
+    BHIPCHK(b, hipMemsetAsync(b->zmap, 0xFF, sizeof(uint64_t) * (size_t)b->zwords, b->stream));
     if ((rc = balloc(b, (void**)&b->d_inst, sizeof(BInst) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_st, sizeof(pdot_solver_state) * count))) return rc;
     if ((rc = balloc(b, (void**)&b->d_nwork, sizeof(int32_t) * 4))) return rc;
@@ -786,7 +844,7 @@ void pdot_batch_destroy(pdot_batch* b) {
     bfree_graphs(b);
     for (hipEvent_t e : b->events) (void)hipEventDestroy(e);
     void* ps[] = {b->mat, b->vec, b->part, b->red, b->d_inst, b->d_st, b->d_tiles, b->d_chunks,
-                  b->d_nwork, b->d_active, b->d_actlist, b->d_evalout, b->stage};
+                  b->d_nwork, b->d_active, b->d_actlist, b->d_evalout, b->stage, b->zmap};
     for (void* q : ps)
         if (q) (void)hipFree(q);
     if (b->h_st) (void)hipHostFree(b->h_st);
@@ -1041,6 +1099,7 @@ int pdot_batch_download(pdot_batch* b, int32_t k, int32_t which, double* X, doub
         slot = st.avg_slot;
     } else return bfail(b, PDOT_E_ARG, "pdot_batch_download: unknown point");
     const BInst& in = b->inst[k];
+    if (X && (rc = bdensify(b, k, slot))) return rc;
     if (X) {
         if (b->storage == PDOT_STORAGE_F64) {
             BHIPCHK(b, hipMemcpy2DAsync(X, in.n * 8, static_cast<double*>(b->mat) + in.X[slot], in.ld * 8, in.n * 8, in.m,
@@ -1119,6 +1178,26 @@ int pdot_batch_set_defer(pdot_batch* b, int32_t enable) {
         b->h_st[k].defer_ok = enable;
     }
     return bwrite_states(b);
+}
+
+int pdot_batch_set_sparse(pdot_batch* b, int32_t enable) {
+    if (!b) return PDOT_E_ARG;
+    if (enable != 0 && enable != 1) return bfail(b, PDOT_E_ARG, "pdot_batch_set_sparse: enable must be 0 or 1");
+    if ((enable != 0) == b->sparse) return PDOT_OK;
+    BHIPCHK(b, hipSetDevice(b->device));
+    int rc;
+    if (enable) {
+        // dense passes kept no maps: every bit set, then kept from here on
+        BHIPCHK(b, hipMemsetAsync(b->zmap, 0xFF, sizeof(uint64_t) * (size_t)b->zwords, b->stream));
+    } else {
+        for (int32_t k = 0; k < b->count; ++k)
+            for (int s = 0; s < 4; ++s)
+                if ((rc = bdensify(b, k, s))) return rc;
+    }
+    BHIPCHK(b, hipStreamSynchronize(b->stream));
+    bfree_graphs(b);  // the captured passes carry the mode
+    b->sparse = enable != 0;
+    return PDOT_OK;
 }
 
 int pdot_batch_set_launch(pdot_batch* b, int32_t cols_per_tile, int32_t grid_mult) {

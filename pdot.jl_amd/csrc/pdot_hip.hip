@@ -450,57 +450,6 @@ __global__ void widen_kernel(const T* __restrict__ src, double* __restrict__ dst
     }
 }
 
-// Geometry of the zero maps: word ((rb * Cb + cb) * waves + w) * tc + t
-// covers, in column j of tile column t (j = cb + t Cb interleaved, else
-// cb TC + t), the vec rows row0 + (w * 64 + L) * vec of lane L, row0 =
-// rb * TRB, for the lanes below the row block's end.
-struct ZGeom {
-    int64_t n = 0, m = 0;
-    int64_t ld = 0;
-    int32_t R = 0, Cb = 0, TRB = 0, TC = 0, interleave = 0, waves = 0, vec = 0;
-    int64_t tc = 0;     // words per wave and tile
-    int64_t words = 0;  // R * Cb * waves * tc
-};
-
-// Expands every wave segment of a slot to the dense layout in place: lane L
-// takes its 16 bytes from position popcount(bits below L) of the segment, or
-// +0.0 where its bit is clear, so that the slot's memory holds every value in
-// column-major order (the map must then be set to all ones, which describes
-// that layout).  One wave per word; every lane's load completes before any
-// lane of the wave stores.
-template <typename T>
-__global__ __launch_bounds__(256) void densify_kernel(T* __restrict__ X, const uint64_t* __restrict__ z, ZGeom g) {
-    using V = typename VecT<T>::type;
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t nw = (int64_t)gridDim.x * (blockDim.x / kWave);
-    for (int64_t wi = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; wi < g.words; wi += nw) {
-        const int64_t t = wi % g.tc;
-        int64_t r = wi / g.tc;
-        const int64_t w = r % g.waves;
-        r /= g.waves;
-        const int64_t cb = r % g.Cb, rb = r / g.Cb;
-        int64_t ncols, j;
-        if (g.interleave) {
-            ncols = (g.m - cb + g.Cb - 1) / g.Cb;
-            j = cb + t * g.Cb;
-        } else {
-            const int64_t j0 = cb * g.TC;
-            ncols = ((j0 + g.TC < g.m) ? j0 + g.TC : g.m) - j0;
-            j = j0 + t;
-        }
-        const int64_t row0 = rb * g.TRB;
-        const int64_t row_end = (row0 + g.TRB < g.n) ? row0 + g.TRB : g.n;
-        const int64_t seg = j * g.ld + row0 + w * kWave * g.vec;  // wave-uniform
-        const uint64_t word = z[wi];
-        if (t >= ncols || word == ~0ull) continue;  // wave-uniform: past the tile, or dense already
-        const bool active = row0 + (w * kWave + lane) * g.vec < row_end;
-        V v = {};
-        if (active && lane_bit(word, lane)) v = *reinterpret_cast<const V*>(X + seg + (int64_t)slot_in_segment(word) * g.vec);
-        wait_memory_ops();
-        if (active) *reinterpret_cast<V*>(X + seg + (int64_t)lane * g.vec) = v;
-    }
-}
-
 // Lanes marked as holding a value (set bits of the lanes below each row
 // block's end), summed into *out.
 __global__ __launch_bounds__(256) void zcount_kernel(const uint64_t* __restrict__ z, ZGeom g,
@@ -523,11 +472,6 @@ __global__ __launch_bounds__(256) void zcount_kernel(const uint64_t* __restrict_
     }
     for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
     if ((threadIdx.x & (kWave - 1)) == 0 && live) atomicAdd(out, live);
-}
-
-__global__ __launch_bounds__(256) void fill_u64_kernel(uint64_t* __restrict__ z, int64_t count, uint64_t value) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
-        z[i] = value;
 }
 
 }  // namespace

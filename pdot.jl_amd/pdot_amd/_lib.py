@@ -144,6 +144,7 @@ SIGNATURES = {
     "pdot_batch_time_steps": (ctypes.c_int, [_ctx_p, ctypes.c_int64, _c_double_p, _c_double_p]),
     "pdot_batch_use_graphs": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_batch_set_defer": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
+    "pdot_batch_set_sparse": (ctypes.c_int, [_ctx_p, ctypes.c_int32]),
     "pdot_batch_set_launch": (ctypes.c_int, [_ctx_p, ctypes.c_int32, ctypes.c_int32]),
     "pdot_batch_get_launch": (ctypes.c_int, [_ctx_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
     "pdot_batch_device_bytes": (ctypes.c_int64, [_ctx_p]),
@@ -381,6 +382,7 @@ class BatchSolver:
         if self.count == 0 or len(self.ms) != self.count:
             raise ValueError("need the same, non-zero number of row and column sizes")
         self.storage = int(storage)
+        self.sparse = True
         na, npp = _i64(self.ns)
         ma, mp = _i64(self.ms)
         h = ctypes.c_void_p()
@@ -477,6 +479,11 @@ class BatchSolver:
     def set_defer(self, enable: bool) -> None:
         """Deferred averaging on (default) or off; results are identical either way."""
         self._check(self.lib.pdot_batch_set_defer(self.h, int(bool(enable))), "pdot_batch_set_defer")
+
+    def set_sparse(self, enable: bool) -> None:
+        """Zero maps on (default) or off for every instance; results are identical."""
+        self._check(self.lib.pdot_batch_set_sparse(self.h, int(bool(enable))), "pdot_batch_set_sparse")
+        self.sparse = bool(enable)
 
     def set_launch(self, cols_per_tile: int, grid_mult: int = 1) -> None:
         self._check(self.lib.pdot_batch_set_launch(self.h, int(cols_per_tile), int(grid_mult)), "pdot_batch_set_launch")

@@ -36,9 +36,13 @@ class BatchProfile:
 def autotune(batch: _lib.BatchSolver, candidates=((32, 1), (64, 1), (128, 1), (256, 1), (64, 2), (128, 2)),
              steps: int = 8, restore_start=None) -> dict:
     """Pick the step kernel's tile width and grid multiple by timing each
-    candidate on the batch itself (a few passes from the zero start); leaves
-    the fastest applied.  restore_start(batch) re-initialises the iterate
-    afterwards (default: zero start with step 1, weight 1)."""
+    candidate on the batch itself (a few passes from the zero start, with the
+    zero maps off so that every element is read and written, as in the dense
+    phase of a solve); leaves the fastest applied.  restore_start(batch)
+    re-initialises the iterate afterwards (default: zero start with step 1,
+    weight 1)."""
+    sparse = batch.sparse
+    batch.set_sparse(False)
     results = []
     for cols, mult in candidates:
         batch.set_launch(cols, mult)
@@ -49,6 +53,7 @@ def autotune(batch: _lib.BatchSolver, candidates=((32, 1), (64, 1), (128, 1), (2
                         "step_kernel_ms": ks / steps, **batch.get_launch()})
     best = min(results, key=lambda r: r["ms_per_step"])
     batch.set_launch(best["cols_per_tile"], best["grid_mult"])
+    batch.set_sparse(sparse)
     if restore_start is not None:
         restore_start(batch)
     return {"best": best, "trials": results}

@@ -17,6 +17,7 @@ import pytest
 
 import pdot_amd as P
 from pdot_amd import _lib, instances
+from pdot_amd.batch import optimize_batch
 from test_gpu_kernel_choices import _same_solves
 
 pytestmark = pytest.mark.gpu
@@ -121,3 +122,34 @@ def test_zero_share_counts_the_zero_units(storage):
     assert 0.3 < share < 1.0
     assert share == pytest.approx(units_zero.mean(), rel=0, abs=1e-12)
     dev.close()
+
+
+def _batch_solve(probs, storage, params, sparse):
+    b = _lib.BatchSolver([p.num_row for p in probs], [p.num_col for p in probs], storage=storage)
+    for k, p in enumerate(probs):
+        b.upload(k, p.cost_matrix, p.source_distribution, p.target_distribution)
+    b.set_sparse(sparse)
+    outs = optimize_batch(params, probs, storage=storage, batch=b)
+    b.close()
+    return outs
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_batch_zero_maps_agree_bitwise(storage):
+    """The batched passes with zero maps (the default) against the same
+    passes with the maps off, over uneven instances that finish at different
+    times (the column tiling changes as instances leave)."""
+    sizes = [(300, 257), (1, 40), (50, 1), (700, 650), (130, 1500), (64, 64)]
+    probs = [instances.gaussian_shift(n, m, seed=40 + k) for k, (n, m) in enumerate(sizes)]
+    params = P.default_parameters(1e-5)
+    on, off = (_batch_solve(probs, storage, params, s) for s in (True, False))
+    for a, b in zip(on, off):
+        assert a.termination_reason == P.TerminationReason.TERMINATION_REASON_OPTIMAL
+        assert b.iteration_count == a.iteration_count
+        assert [s.restart_used for s in a.iteration_stats] == [s.restart_used for s in b.iteration_stats]
+        for sa, sb in zip(a.iteration_stats, b.iteration_stats):
+            assert sa.convergence_information[0].primal_objective == sb.convergence_information[0].primal_objective
+        assert np.array_equal(a.primal_solution, b.primal_solution)
+        assert np.array_equal(a.dual_source_solution, b.dual_source_solution)
+        assert np.array_equal(a.dual_target_solution, b.dual_target_solution)
+    assert max(o.iteration_count for o in on) > 4 * min(o.iteration_count for o in on)
