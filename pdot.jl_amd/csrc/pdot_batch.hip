@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kBBlock) void bstep_kernel(BParams p) {
     __shared__ BInst sinst;
     __shared__ BTile stl;
     __shared__ double red[kBBlock];
-    __shared__ double colbuf[kBWaves][kChunk];
+    __shared__ double colbuf[kBWaves * col_rows<T>()][kChunk];
     const int tid = threadIdx.x;
     const int ntiles = p.nwork[0];
     for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {

@@ -182,6 +182,26 @@ __device__ __forceinline__ double readlane_f64(double x, int lane) {
 
 // Sum over the 64 lanes of a wave in a fixed DPP tree; every lane gets the
 // total.  Must be called with all lanes active.
+// Sum over each 16-lane row of a wave (the first four steps of wave_sum's
+// tree); every lane of a row gets its row's total.  All lanes active.
+constexpr int kWaveRows = 4;
+__device__ __forceinline__ double row_sum(double x) {
+    x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
+    x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
+    x += dpp_f64<0x141, 0xF>(x);  // row_half_mirror
+    x += dpp_f64<0x140, 0xF>(x);  // row_mirror
+    return x;
+}
+// The wave total from the four row totals, added in the order wave_sum's last
+// two steps add them: (r3 + r2) + (r1 + r0).
+__device__ __forceinline__ double rows_total(const double* r) {
+    return (r[3] + r[2]) + (r[1] + r[0]);
+}
+// LDS rows of column sums per wave in step_tile: FP64 storage leaves the last
+// two steps of the tree to the flush (fewer vector instructions per column);
+// FP32 storage reduces each column to the wave total (measured faster there).
+template <typename T>
+constexpr int col_rows() { return sizeof(T) == 8 ? kWaveRows : 1; }
 __device__ __forceinline__ double wave_sum(double x) {
     x += dpp_f64<0xB1, 0xF>(x);   // quad_perm [1,0,3,2]
     x += dpp_f64<0x4E, 0xF>(x);   // quad_perm [2,3,0,1]
@@ -704,8 +724,9 @@ struct TileArgs {
 // One PDHG pass over a tile (src/solver_gpu.jl:125-174 and 237-264,
 // src/saddle_point_gpu.jl:115-135): each lane owns VEC consecutive rows and
 // walks the tile's columns.  Row sums accumulate in registers, column sums
-// are reduced per column across the wave (DPP) and across the block's waves
-// (LDS, every kChunk columns).  With EVAL the same loads also produce the
+// are reduced per column across the wave (DPP; FP64 storage: across each
+// 16-lane row, the rows' totals added at the flush) and across the block's
+// waves (LDS, every kChunk columns; colbuf holds WAVES * col_rows<T>() rows).  With EVAL the same loads also produce the
 // evaluation reductions of the input point and of its average.  Called by
 // all threads of the block; thread 0 receives ||Delta||^2 of the tile in
 // *dd_out and the kMatStats evaluation reductions in ev_out.  NT: the
@@ -933,17 +954,38 @@ __device__ __forceinline__ void step_tile(const TileArgs<T>& a, double (*colbuf)
                     if (!skip_avg) zan[tb + q] = ban;
                 }
             }
-            const double s = wave_sum(csl);
-            if (colok && lane == 0) colbuf[w][tb + q - tc] = s;
+            if constexpr (col_rows<T>() == kWaveRows) {
+                // row totals to LDS; the flush finishes each wave's total with
+                // the additions wave_sum would make
+                const double s = row_sum(csl);
+                if (colok && (lane & 15) == 0) colbuf[w * kWaveRows + (lane >> 4)][tb + q - tc] = s;
+            } else {
+                const double s = wave_sum(csl);
+                if (colok && lane == 0) colbuf[w][tb + q - tc] = s;
+            }
         }
         // flush the column sums after the chunk's last batch (block-uniform)
         const int64_t tend = (tc + kChunk < ncols) ? tc + kChunk : ncols;
         if (tb + UNROLL >= tend) {
             __syncthreads();
             if (do_step && tid < tend - tc) {
-                double t = colbuf[0][tid];
+                double t;
+                if constexpr (col_rows<T>() == kWaveRows) {
+                    double r[kWaveRows];
 #pragma unroll
-                for (int ww = 1; ww < WAVES; ++ww) t += colbuf[ww][tid];
+                    for (int k = 0; k < kWaveRows; ++k) r[k] = colbuf[k][tid];
+                    t = rows_total(r);
+#pragma unroll
+                    for (int ww = 1; ww < WAVES; ++ww) {
+#pragma unroll
+                        for (int k = 0; k < kWaveRows; ++k) r[k] = colbuf[ww * kWaveRows + k][tid];
+                        t += rows_total(r);
+                    }
+                } else {
+                    t = colbuf[0][tid];
+#pragma unroll
+                    for (int ww = 1; ww < WAVES; ++ww) t += colbuf[ww][tid];
+                }
                 a.colpart[j0 + (tc + tid) * jstep] = t;
             }
             __syncthreads();

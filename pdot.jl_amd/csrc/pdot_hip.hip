@@ -121,7 +121,7 @@ __global__ __launch_bounds__((StepShape<T, COST != kCostDense>::kWaves * kWave))
     constexpr int TR = WAVES * kWave * VecT<T>::N;
     __shared__ pdot_solver_state sst;
     __shared__ double red[WAVES * kWave];
-    __shared__ double colbuf[WAVES][kChunk];
+    __shared__ double colbuf[WAVES * col_rows<T>()][kChunk];
     const int tid = threadIdx.x;
     if (tid == 0) sst = *p.st;
     __syncthreads();
