@@ -11,7 +11,10 @@ off, so everything must agree bit for bit:
   switched off and on again;
 * the share of entries the maps skip equals the share of 16-byte units of
   the downloaded iterate that are +0.0 (the maps record exactly the zeros
-  of a point written with them on)."""
+  of a point written with them on);
+* the batched passes (pdot_batch_set_sparse): whole batch solves, and
+  iterates downloaded across restarts, a change of column tiling and a
+  toggle of the maps."""
 import numpy as np
 import pytest
 
@@ -153,3 +156,51 @@ def test_batch_zero_maps_agree_bitwise(storage):
         assert np.array_equal(a.dual_source_solution, b.dual_source_solution)
         assert np.array_equal(a.dual_target_solution, b.dual_target_solution)
     assert max(o.iteration_count for o in on) > 4 * min(o.iteration_count for o in on)
+
+
+@pytest.mark.parametrize("storage", [_lib.STORAGE_F64, _lib.STORAGE_F32])
+def test_batch_iterates_agree_across_restarts_tilings_and_toggles(storage):
+    """Batched iterates with the maps on and off, downloaded mid-solve: after
+    a restart to the average with lagging contributions, a change of column
+    tiling (the maps follow the fixed row tiles, not the columns) and the
+    maps switched off (every slot expanded in place) and on again."""
+    sizes = [(300, 257), (700, 650), (64, 900)]
+    probs = [instances.gaussian_shift(n, m, seed=60 + k) for k, (n, m) in enumerate(sizes)]
+    bs = []
+    for sparse in (True, False):
+        b = _lib.BatchSolver([n for n, _ in sizes], [m for _, m in sizes], storage=storage)
+        for k, p in enumerate(probs):
+            b.upload(k, p.cost_matrix, p.source_distribution, p.target_distribution)
+        b.set_sparse(sparse)
+        b.set_start(_lib.POLICY_ADAPTIVE, 2.0, 1.0, 1.0, [1.0, 0.5, 2.0])
+        bs.append(b)
+
+    def same(average=True):
+        for k in range(len(sizes)):
+            for which in ((_lib.POINT_CURRENT, _lib.POINT_AVERAGE) if average else (_lib.POINT_CURRENT,)):
+                xs, ys = (b.download(k, which) for b in bs)
+                for x, y in zip(xs, ys):
+                    assert np.array_equal(x, y)
+        return bs[0].download(1, _lib.POINT_CURRENT)[0]
+
+    for b in bs:
+        b.run_steps([41, 37, 30])
+    same()
+    for b in bs:
+        b.restart([2, 2, 1], [0.7, 1.3, 1.0])
+        b.run_steps([20, 21, 22])
+    same()
+    for b in bs:
+        b.set_launch(32, 1)
+        b.run_steps([15, 15, 15])
+    same()
+    bs[0].set_sparse(False)
+    for b in bs:
+        b.run_steps([5, 4, 3])
+    bs[0].set_sparse(True)
+    for b in bs:
+        b.run_steps([11, 12, 13])
+    X = same()
+    for b in bs:
+        b.close()
+    assert (X == 0).mean() > 0.5
